@@ -1,0 +1,169 @@
+/*
+ * rdmce: maximal clique enumeration on AMD Instinct MI355X (gfx950).
+ *
+ * C ABI of librdmce.so.  Plain pointers and sizes only; every call returns an
+ * mce_status (0 = MCE_OK) and never exits the process - the message of the
+ * last failure on the calling thread is returned by mce_last_error().
+ *
+ * Ownership: inputs are caller-owned and only read during the call; graphs,
+ * plans and results are library-owned and released with their *_free call.
+ * Device memory is allocated and released by the library.  One solve per
+ * device at a time; calls on different devices may run concurrently.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference repository root).
+ */
+#ifndef RDMCE_H
+#define RDMCE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t mce_status;
+#define MCE_OK 0
+#define MCE_ERR_ARG 1          /* invalid argument */
+#define MCE_ERR_IO 2           /* file could not be opened / read / written */
+#define MCE_ERR_NOMEM 3        /* host or device allocation failed */
+#define MCE_ERR_DEVICE 4       /* HIP runtime error, or no device */
+#define MCE_ERR_UNSUPPORTED 5  /* input beyond a documented limit */
+#define MCE_ERR_INTERNAL 6
+
+/* Vertex orders: OrderType, RDMCE/include/graph.h:4-10 */
+#define MCE_ORDER_UNCHANGED 0
+#define MCE_ORDER_ASC 1
+#define MCE_ORDER_DES 2
+#define MCE_ORDER_DEG 3
+
+/* Root granularity: LEVEL_2 (edge roots, reference default) or LEVEL_1
+ * (vertex roots), RDMCE/include/mce_gpu.cuh:4 and
+ * RDMCE/src/kernels/BkPivotBitBalance.cuh:414-470 */
+#define MCE_ROOT_VERTEX 1
+#define MCE_ROOT_EDGE 2
+
+/* Root schedules (dispatch order of roots; never changes results). */
+#define MCE_SCHED_PLAIN 0    /* reverse edge-slot order, BkPivotBitBalance.cuh:429-431 */
+#define MCE_SCHED_CUSTOM 1   /* caller-supplied root list (mce_options.roots) */
+
+const char *mce_last_error(void);
+const char *mce_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Host graph  (class Graph, RDMCE/include/graph.h:12-52)                    */
+/* ------------------------------------------------------------------------ */
+typedef struct mce_graph mce_graph;
+
+typedef struct mce_graph_info {
+  uint64_t n;            /* GetNumVertices  graph.h:39 */
+  uint64_t m;            /* GetNumEdges     graph.h:40 (undirected) */
+  uint64_t max_degree;   /* GetMaxDegree    graph.h:41 */
+  uint64_t degeneracy;   /* GetDegeneracy   graph.h:42 */
+} mce_graph_info;
+
+/* Graph::LoadFromFile, RDMCE/src/graph.cc:64-152 (COO text: "u v" per line,
+ * unparseable lines skipped, self loops dropped, duplicates merged, used ids
+ * relabelled densely in ascending order). */
+mce_status mce_graph_load_coo_text(const char *path, mce_graph **out);
+/* Graph::LoadFromBin, RDMCE/src/graph.cc:178-213. */
+mce_status mce_graph_load_bin(const char *path, mce_graph **out);
+/* Graph::StoreIntoBin, RDMCE/src/graph.cc:154-176.  Writes exactly `path`
+ * (mce_graph_bin_path gives the reference's extension-replacing name). */
+mce_status mce_graph_store_bin(const mce_graph *g, const char *path);
+mce_status mce_graph_bin_path(const char *input_path, char *buf, uint64_t buf_len);
+/* Reference loader semantics applied to in-memory endpoint pairs. */
+mce_status mce_graph_from_pairs(const uint32_t *u, const uint32_t *v, uint64_t count, mce_graph **out);
+/* Copies a CSR (rows sorted ascending, symmetric, no self loops).  labels may
+ * be NULL (identity).  degeneracy 0 = use max degree. */
+mce_status mce_graph_from_csr(uint64_t n, const uint64_t *rowoff, const uint32_t *col, const uint32_t *labels,
+                              uint64_t degeneracy, mce_graph **out);
+/* Graph::SortByOrder, RDMCE/src/graph.cc:241-306. */
+mce_status mce_graph_order(mce_graph *g, int32_t order);
+mce_status mce_graph_get_info(const mce_graph *g, mce_graph_info *info);
+/* Graph::GetRowOffset/GetColIdx (graph.h:43-44) plus labels; pointers stay
+ * valid until the graph is modified or freed.  Offsets are 64-bit. */
+mce_status mce_graph_get_csr(const mce_graph *g, const uint64_t **rowoff, const uint32_t **col,
+                             const uint32_t **labels);
+/* Core number per vertex (computed on first use). */
+mce_status mce_graph_get_core(mce_graph *g, const uint32_t **core);
+mce_status mce_graph_write_coo_text(const mce_graph *g, const char *path);
+void mce_graph_free(mce_graph *g);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic generators (no reference counterpart; deterministic per seed)   */
+/* ------------------------------------------------------------------------ */
+mce_status mce_gen_er(uint32_t n, double p, uint64_t seed, mce_graph **out);
+mce_status mce_gen_rmat(int32_t scale, uint64_t edge_factor, double a, double b, double c, uint64_t seed,
+                        mce_graph **out);
+mce_status mce_gen_chung_lu_planted(uint64_t n, double gamma, double mean_degree, uint32_t n_comm, uint32_t cmin,
+                                    uint32_t cmax, double p_in, uint64_t seed, mce_graph **out);
+mce_status mce_gen_batch_params(uint64_t seed, uint64_t index, uint32_t nmin, uint32_t nmax, double pmin,
+                                double pmax, uint32_t *n, double *p, uint64_t *edge_seed);
+
+/* ------------------------------------------------------------------------ */
+/* Solver                                                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct mce_options {
+  int32_t device;            /* HIP device ordinal (MceGpuSolver(g, device_id), mce_gpu.cu:9-12) */
+  int32_t root_mode;         /* MCE_ROOT_EDGE (default) or MCE_ROOT_VERTEX */
+  int32_t waves_per_cu;      /* resident wavefronts per CU; 0 = default (WARP_PER_SM analogue) */
+  int32_t share_bound;       /* TASK_SHARE_BOUND: |P| at which a level is exposed; 0 = 24; <0 = off */
+  int32_t want_list;         /* 1 = return the clique list */
+  int32_t want_root_stats;   /* 1 = return per-root node counts and wave cycles */
+  int32_t schedule;          /* MCE_SCHED_* */
+  int32_t lds_tier;          /* 1 = stage small subproblems in LDS (default), 0 = HBM only */
+  uint64_t list_capacity;    /* u32 entries reserved for the clique list; 0 = 1<<24 */
+  uint64_t slot_bytes;       /* per-wave HBM scratch; 0 = sized automatically */
+  const uint64_t *roots;     /* MCE_SCHED_CUSTOM: root descriptors in dispatch order */
+  uint64_t n_roots;          /*   (see mce_plan_roots for their encoding) */
+  uint64_t reserved[8];
+} mce_options;
+
+void mce_options_init(mce_options *opt);
+
+typedef struct mce_result {
+  uint64_t count;            /* number of maximal cliques (MC numbers, mce_gpu.cu:20) */
+  uint64_t hash;             /* sum over cliques of splitmix64(sum of splitmix64(label)) */
+  uint32_t hist_len;         /* hist[k] = cliques of size k, k < hist_len */
+  uint64_t *hist;
+  uint64_t list_len;         /* u32 entries: [k, label_1 .. label_k]* (unsorted) */
+  uint32_t *list;
+  uint64_t n_roots;          /* canonical roots (edge: m, vertex: n) */
+  uint64_t *root_nodes;      /* per canonical root: search-tree nodes (want_root_stats) */
+  uint64_t *root_cycles;     /* per canonical root: wave cycles spent (want_root_stats) */
+  uint64_t total_nodes;
+  uint64_t deferred_roots;   /* roots re-run in the large-scratch pass */
+  uint64_t shared_tasks;     /* subtrees handed between wavefronts */
+  double kernel_ms;          /* device time of the enumeration kernels (HIP events) */
+  double total_ms;           /* wall time of the call */
+  int32_t passes;
+  int32_t waves;             /* wavefronts launched */
+  uint64_t reserved[8];
+} mce_result;
+
+/* BkSolverWrapper(Graph&, device_id), BkPivotBitBalance.cuh:611-678 and
+ * MceGpuSolver::Solve, mce_gpu.cu:14-22: upload, enumerate, free. */
+mce_status mce_solve(const mce_graph *g, const mce_options *opt, mce_result **out);
+/* Same on a caller CSR (must be in an order where every vertex has at most
+ * `degeneracy` higher-ordered neighbours when degeneracy != 0). */
+mce_status mce_solve_csr(uint64_t n, const uint64_t *rowoff, const uint32_t *col, const uint32_t *labels,
+                         uint64_t degeneracy, const mce_options *opt, mce_result **out);
+void mce_result_free(mce_result *r);
+
+/* Device-resident plan: the CSR, root list and scratch stay in HBM, so that
+ * mce_plan_run measures enumeration only (inputs resident). */
+typedef struct mce_plan mce_plan;
+mce_status mce_plan_create(const mce_graph *g, const mce_options *opt, mce_plan **out);
+mce_status mce_plan_run(mce_plan *p, mce_result **out);
+/* Canonical root descriptors (edge roots: v0 << 32 | v1 with v0 < v1 in the
+ * graph's order, sorted ascending; vertex roots: v).  Host copy. */
+mce_status mce_plan_roots(const mce_graph *g, int32_t root_mode, uint64_t *out, uint64_t cap, uint64_t *count);
+void mce_plan_free(mce_plan *p);
+
+int32_t mce_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RDMCE_H */

@@ -1,0 +1,12 @@
+"""rdmce_amd: maximal clique enumeration on AMD Instinct MI355X.
+
+Host-side mirror of the reference's interface (``Graph``, ``MceGpuSolver``,
+``BkSolverWrapper``) over the C ABI of librdmce.so, whose HIP kernels do all
+the enumeration.
+"""
+from ._lib import (MceError, ROOT_EDGE, ROOT_VERTEX, LIB_PATH, lib)  # noqa: F401
+from .graph import Graph, OrderType, batch_graph_params, bin_path_for  # noqa: F401
+from .solver import BkSolverWrapper, MceGpuSolver, MceOutput, Plan, device_count, solve  # noqa: F401
+
+__all__ = ["Graph", "OrderType", "MceGpuSolver", "BkSolverWrapper", "solve", "Plan", "MceOutput", "MceError",
+           "ROOT_EDGE", "ROOT_VERTEX", "batch_graph_params", "bin_path_for", "device_count", "lib", "LIB_PATH"]

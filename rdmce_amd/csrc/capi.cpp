@@ -1,0 +1,674 @@
+// C ABI of librdmce.so (declared in include/rdmce.h): host graph handling,
+// generators, and the device plan that uploads a graph once and runs the
+// enumeration kernels on it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/rdmce.h"
+#include "generators.hpp"
+#include "host_graph.hpp"
+#include "mce_device.hpp"
+#include "parallel.hpp"
+#include "rng.hpp"
+
+using namespace rdmce;
+
+struct mce_graph {
+  HostGraph g;
+};
+
+namespace {
+thread_local std::string g_err;
+
+mce_status fail(mce_status s, const std::string &msg) {
+  g_err = msg;
+  return s;
+}
+
+#define HIP_TRY(expr)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return fail(MCE_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));            \
+  } while (0)
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+template <class F>
+mce_status guarded(F &&f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc &) {
+    return fail(MCE_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception &e) {
+    return fail(MCE_ERR_INTERNAL, e.what());
+  }
+}
+
+mce_status new_graph(mce_graph **out) {
+  if (!out) return fail(MCE_ERR_ARG, "null output pointer");
+  *out = new mce_graph();
+  return MCE_OK;
+}
+
+// Canonical roots: edge roots (v0 << 32 | v1), v0 < v1, ascending; vertex
+// roots v ascending.
+void build_roots(const HostGraph &g, int mode, std::vector<uint64_t> &roots, uint64_t &max_out) {
+  std::vector<uint64_t> first(g.n + 1, 0), outdeg(g.n + 1, 0);
+  max_out = 0;
+  for (uint64_t v = 0; v < g.n; ++v) {
+    const uint32_t *b = g.col.data() + g.rowoff[v], *e = g.col.data() + g.rowoff[v + 1];
+    first[v] = (uint64_t)(std::upper_bound(b, e, (uint32_t)v) - b);
+    outdeg[v] = (uint64_t)(e - b) - first[v];
+    max_out = std::max(max_out, outdeg[v]);
+  }
+  if (mode == MCE_ROOT_VERTEX) {
+    roots.resize(g.n);
+    for (uint64_t v = 0; v < g.n; ++v) roots[v] = v;
+    return;
+  }
+  std::vector<uint64_t> off(g.n + 1, 0);
+  for (uint64_t v = 0; v < g.n; ++v) off[v + 1] = off[v] + outdeg[v];
+  roots.resize(off[g.n]);
+  parallel_chunks(g.n, [&](int, uint64_t b, uint64_t e) {
+    for (uint64_t v = b; v < e; ++v) {
+      const uint32_t *nb = g.col.data() + g.rowoff[v] + first[v];
+      for (uint64_t k = 0; k < outdeg[v]; ++k) roots[off[v] + k] = (v << 32) | nb[k];
+    }
+  });
+}
+}  // namespace
+
+extern "C" {
+
+const char *mce_last_error(void) { return g_err.c_str(); }
+const char *mce_version(void) { return "rdmce-mi355x 0.1"; }
+
+// ---------------------------------------------------------------------------
+// Graph
+// ---------------------------------------------------------------------------
+mce_status mce_graph_load_coo_text(const char *path, mce_graph **out) {
+  return guarded([&] {
+    if (!path) return fail(MCE_ERR_ARG, "null path");
+    mce_status s = new_graph(out);
+    if (s) return s;
+    std::string err;
+    if (!load_coo_text((*out)->g, path, &err)) {
+      delete *out;
+      *out = nullptr;
+      return fail(MCE_ERR_IO, err);
+    }
+    return MCE_OK;
+  });
+}
+
+mce_status mce_graph_load_bin(const char *path, mce_graph **out) {
+  return guarded([&] {
+    if (!path) return fail(MCE_ERR_ARG, "null path");
+    mce_status s = new_graph(out);
+    if (s) return s;
+    std::string err;
+    if (!load_bin((*out)->g, path, &err)) {
+      delete *out;
+      *out = nullptr;
+      return fail(MCE_ERR_IO, err);
+    }
+    return MCE_OK;
+  });
+}
+
+mce_status mce_graph_store_bin(const mce_graph *g, const char *path) {
+  return guarded([&] {
+    if (!g || !path) return fail(MCE_ERR_ARG, "null argument");
+    std::string err;
+    if (!store_bin(g->g, path, &err)) return fail(MCE_ERR_IO, err);
+    return MCE_OK;
+  });
+}
+
+mce_status mce_graph_bin_path(const char *input_path, char *buf, uint64_t buf_len) {
+  if (!input_path || !buf) return fail(MCE_ERR_ARG, "null argument");
+  std::string s = bin_path_for(input_path);
+  if (s.size() + 1 > buf_len) return fail(MCE_ERR_ARG, "buffer too small");
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return MCE_OK;
+}
+
+mce_status mce_graph_from_pairs(const uint32_t *u, const uint32_t *v, uint64_t count, mce_graph **out) {
+  return guarded([&] {
+    if ((!u || !v) && count) return fail(MCE_ERR_ARG, "null endpoint array");
+    mce_status s = new_graph(out);
+    if (s) return s;
+    graph_from_pairs((*out)->g, u, v, count);
+    return MCE_OK;
+  });
+}
+
+mce_status mce_graph_from_csr(uint64_t n, const uint64_t *rowoff, const uint32_t *col, const uint32_t *labels,
+                              uint64_t degeneracy, mce_graph **out) {
+  return guarded([&] {
+    if (!rowoff || (!col && rowoff[n])) return fail(MCE_ERR_ARG, "null CSR array");
+    if (n >= (1ull << 32)) return fail(MCE_ERR_UNSUPPORTED, "more than 2^32-1 vertices");
+    for (uint64_t v = 0; v < n; ++v) {
+      if (rowoff[v + 1] < rowoff[v]) return fail(MCE_ERR_ARG, "row offsets decrease");
+      for (uint64_t i = rowoff[v]; i < rowoff[v + 1]; ++i) {
+        if (col[i] >= n || col[i] == v) return fail(MCE_ERR_ARG, "column id out of range or self loop");
+        if (i > rowoff[v] && col[i] <= col[i - 1]) return fail(MCE_ERR_ARG, "row not strictly ascending");
+      }
+    }
+    if (rowoff[n] % 2) return fail(MCE_ERR_ARG, "odd number of adjacency entries");
+    mce_status s = new_graph(out);
+    if (s) return s;
+    HostGraph &g = (*out)->g;
+    g.n = n;
+    g.m = rowoff[n] / 2;
+    g.rowoff.assign(rowoff, rowoff + n + 1);
+    g.col.assign(col, col + rowoff[n]);
+    g.labels.resize(n);
+    for (uint64_t v = 0; v < n; ++v) g.labels[v] = labels ? labels[v] : (uint32_t)v;
+    g.recompute_max_degree();
+    g.degeneracy = degeneracy ? degeneracy : g.max_degree;
+    return MCE_OK;
+  });
+}
+
+mce_status mce_graph_order(mce_graph *g, int32_t order) {
+  return guarded([&] {
+    if (!g) return fail(MCE_ERR_ARG, "null graph");
+    if (order < MCE_ORDER_UNCHANGED || order > MCE_ORDER_DEG) return fail(MCE_ERR_ARG, "unknown order");
+    apply_order(g->g, (Order)order);
+    return MCE_OK;
+  });
+}
+
+mce_status mce_graph_get_info(const mce_graph *g, mce_graph_info *info) {
+  if (!g || !info) return fail(MCE_ERR_ARG, "null argument");
+  info->n = g->g.n;
+  info->m = g->g.m;
+  info->max_degree = g->g.max_degree;
+  info->degeneracy = g->g.degeneracy;
+  return MCE_OK;
+}
+
+mce_status mce_graph_get_csr(const mce_graph *g, const uint64_t **rowoff, const uint32_t **col,
+                             const uint32_t **labels) {
+  if (!g) return fail(MCE_ERR_ARG, "null graph");
+  if (rowoff) *rowoff = g->g.rowoff.data();
+  if (col) *col = g->g.col.data();
+  if (labels) *labels = g->g.labels.data();
+  return MCE_OK;
+}
+
+mce_status mce_graph_get_core(mce_graph *g, const uint32_t **core) {
+  return guarded([&] {
+    if (!g || !core) return fail(MCE_ERR_ARG, "null argument");
+    if (g->g.core.size() != g->g.n) {
+      std::vector<uint32_t> order, c;
+      degeneracy_order(g->g, order, c);
+      g->g.core.swap(c);
+    }
+    *core = g->g.core.data();
+    return MCE_OK;
+  });
+}
+
+mce_status mce_graph_write_coo_text(const mce_graph *g, const char *path) {
+  return guarded([&] {
+    if (!g || !path) return fail(MCE_ERR_ARG, "null argument");
+    std::string err;
+    if (!write_coo_text(g->g, path, &err)) return fail(MCE_ERR_IO, err.empty() ? "write failed" : err);
+    return MCE_OK;
+  });
+}
+
+void mce_graph_free(mce_graph *g) { delete g; }
+
+// ---------------------------------------------------------------------------
+// Generators
+// ---------------------------------------------------------------------------
+mce_status mce_gen_er(uint32_t n, double p, uint64_t seed, mce_graph **out) {
+  return guarded([&] {
+    if (!(p >= 0.0 && p <= 1.0)) return fail(MCE_ERR_ARG, "p outside [0, 1]");
+    std::vector<uint64_t> keys;
+    gen_er(n, p, seed, keys);
+    mce_status s = new_graph(out);
+    if (s) return s;
+    graph_from_keys((*out)->g, std::move(keys));
+    return MCE_OK;
+  });
+}
+
+mce_status mce_gen_rmat(int32_t scale, uint64_t edge_factor, double a, double b, double c, uint64_t seed,
+                        mce_graph **out) {
+  return guarded([&] {
+    if (scale < 1 || scale > 31) return fail(MCE_ERR_ARG, "scale outside [1, 31]");
+    if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0) return fail(MCE_ERR_ARG, "bad quadrant probabilities");
+    std::vector<uint64_t> keys;
+    gen_rmat(scale, edge_factor, a, b, c, seed, keys);
+    mce_status s = new_graph(out);
+    if (s) return s;
+    graph_from_keys((*out)->g, std::move(keys));
+    return MCE_OK;
+  });
+}
+
+mce_status mce_gen_chung_lu_planted(uint64_t n, double gamma, double mean_degree, uint32_t n_comm, uint32_t cmin,
+                                    uint32_t cmax, double p_in, uint64_t seed, mce_graph **out) {
+  return guarded([&] {
+    if (n == 0 || n >= (1ull << 32) || gamma <= 1.0 || mean_degree <= 0 || cmin > cmax || p_in < 0 || p_in > 1)
+      return fail(MCE_ERR_ARG, "bad Chung-Lu parameters");
+    std::vector<uint64_t> keys;
+    gen_chung_lu_planted(n, gamma, mean_degree, n_comm, cmin, cmax, p_in, seed, keys);
+    mce_status s = new_graph(out);
+    if (s) return s;
+    graph_from_keys((*out)->g, std::move(keys));
+    return MCE_OK;
+  });
+}
+
+mce_status mce_gen_batch_params(uint64_t seed, uint64_t index, uint32_t nmin, uint32_t nmax, double pmin,
+                                double pmax, uint32_t *n, double *p, uint64_t *edge_seed) {
+  if (!n || !p || !edge_seed || nmin > nmax || pmin > pmax) return fail(MCE_ERR_ARG, "bad batch parameters");
+  BatchGraphParams bp = batch_graph_params(seed, index, nmin, nmax, pmin, pmax);
+  *n = bp.n;
+  *p = bp.p;
+  *edge_seed = bp.seed;
+  return MCE_OK;
+}
+
+void mce_options_init(mce_options *opt) {
+  if (!opt) return;
+  std::memset(opt, 0, sizeof(*opt));
+  opt->device = 0;
+  opt->root_mode = MCE_ROOT_EDGE;
+  opt->waves_per_cu = 0;
+  opt->share_bound = 0;
+  opt->schedule = MCE_SCHED_PLAIN;
+  opt->lds_tier = 1;
+}
+
+int32_t mce_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Plan
+// ---------------------------------------------------------------------------
+struct mce_plan {
+  mce_options opt{};
+  int device = 0;
+  int cus = 0;
+  uint64_t n = 0;
+  uint64_t n_roots = 0;
+  uint32_t hist_len = 0;
+  uint32_t waves = 0;
+  uint64_t slot_bytes = 0;
+  uint64_t deferred_cap = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint64_t *d_rowoff = nullptr;
+  uint32_t *d_col = nullptr;
+  uint64_t *d_lhash = nullptr;
+  uint32_t *d_labels = nullptr;
+  uint64_t *d_roots = nullptr;
+  uint64_t *d_perm = nullptr;
+  uint8_t *d_slots = nullptr;
+  unsigned long long *d_ctr = nullptr;  // see Ctr
+  unsigned long long *d_hist = nullptr;
+  unsigned long long *d_root_nodes = nullptr;
+  unsigned long long *d_root_cycles = nullptr;
+  uint32_t *d_list = nullptr;
+  uint64_t list_cap = 0;
+  uint64_t *d_deferred = nullptr;
+  uint64_t *d_deferred2 = nullptr;
+  uint8_t *d_slots2 = nullptr;
+  uint64_t slots2_bytes = 0;
+};
+
+namespace {
+enum Ctr { kCount, kHash, kNodes, kNext, kListCursor, kNDeferred, kDeferNeed, kErr, kNumCtr };
+
+template <class T>
+hipError_t dalloc(T **p, uint64_t count) {
+  return hipMalloc((void **)p, std::max<uint64_t>(count, 1) * sizeof(T));
+}
+
+void plan_release(mce_plan *p) {
+  if (!p) return;
+  hipSetDevice(p->device);
+  void *ptrs[] = {p->d_rowoff, p->d_col,       p->d_lhash,       p->d_labels,   p->d_roots,    p->d_perm,
+                  p->d_slots,  p->d_ctr,       p->d_hist,        p->d_root_nodes, p->d_root_cycles,
+                  p->d_list,   p->d_deferred,  p->d_deferred2,   p->d_slots2};
+  for (void *q : ptrs)
+    if (q) hipFree(q);
+  for (auto &e : p->ev)
+    if (e) hipEventDestroy(e);
+  if (p->stream) hipStreamDestroy(p->stream);
+  delete p;
+}
+
+uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+// Scratch need bound of the largest possible root (see need_bound in the
+// kernel): c <= max degree, p <= max out-degree.
+uint64_t worst_need(uint64_t maxdeg, uint64_t maxout) {
+  uint64_t p = std::min(maxdeg, maxout), c = maxdeg, x = maxdeg;
+  uint64_t W = (p + 63) / 64, Wx = (x + 63) / 64;
+  uint64_t bm_off = align_up(4 * c, 256);
+  uint64_t st_off = bm_off + align_up(8 * W * align_up(p + x, 64), 256);
+  return st_off + (p + 1) * (4 * W + Wx + 4) * 8;
+}
+
+mce_status plan_create_impl(const HostGraph &g, const mce_options &opt, mce_plan **out) {
+  auto p = new mce_plan();
+  p->opt = opt;
+  p->device = opt.device;
+  auto bail = [&](mce_status s) {
+    plan_release(p);
+    return s;
+  };
+#define PTRY(expr)                                                                                   \
+  do {                                                                                               \
+    hipError_t e_ = (expr);                                                                          \
+    if (e_ != hipSuccess)                                                                            \
+      return bail(fail(e_ == hipErrorOutOfMemory ? MCE_ERR_NOMEM : MCE_ERR_DEVICE,                   \
+                       std::string(#expr) + ": " + hipGetErrorString(e_)));                          \
+  } while (0)
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return bail(fail(MCE_ERR_DEVICE, "no HIP device"));
+  if (opt.device < 0 || opt.device >= ndev) return bail(fail(MCE_ERR_ARG, "device ordinal out of range"));
+  PTRY(hipSetDevice(opt.device));
+  hipDeviceProp_t prop;
+  PTRY(hipGetDeviceProperties(&prop, opt.device));
+  p->cus = prop.multiProcessorCount;
+  PTRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+  for (auto &e : p->ev) PTRY(hipEventCreate(&e));
+
+  const int mode = opt.root_mode == MCE_ROOT_VERTEX ? MCE_ROOT_VERTEX : MCE_ROOT_EDGE;
+  p->opt.root_mode = mode;
+  std::vector<uint64_t> roots;
+  uint64_t max_out = 0;
+  build_roots(g, mode, roots, max_out);
+  p->n = g.n;
+  p->n_roots = roots.size();
+  p->hist_len = (uint32_t)std::min<uint64_t>(max_out + 3, 1ull << 20);
+  if (p->hist_len < kLdsHist) p->hist_len = kLdsHist;
+
+  std::vector<uint64_t> lhash(g.n);
+  for (uint64_t v = 0; v < g.n; ++v) lhash[v] = splitmix64_mix(g.labels[v]);
+
+  PTRY(dalloc(&p->d_rowoff, g.n + 1));
+  PTRY(dalloc(&p->d_col, g.col.size()));
+  PTRY(dalloc(&p->d_lhash, g.n));
+  PTRY(dalloc(&p->d_labels, g.n));
+  PTRY(dalloc(&p->d_roots, p->n_roots));
+  PTRY(hipMemcpy(p->d_rowoff, g.rowoff.data(), (g.n + 1) * 8, hipMemcpyHostToDevice));
+  if (!g.col.empty()) PTRY(hipMemcpy(p->d_col, g.col.data(), g.col.size() * 4, hipMemcpyHostToDevice));
+  if (g.n) {
+    PTRY(hipMemcpy(p->d_lhash, lhash.data(), g.n * 8, hipMemcpyHostToDevice));
+    PTRY(hipMemcpy(p->d_labels, g.labels.data(), g.n * 4, hipMemcpyHostToDevice));
+  }
+  if (p->n_roots) PTRY(hipMemcpy(p->d_roots, roots.data(), p->n_roots * 8, hipMemcpyHostToDevice));
+  if (opt.schedule == MCE_SCHED_CUSTOM) {
+    if (!opt.roots) return bail(fail(MCE_ERR_ARG, "custom schedule without a root list"));
+    if (opt.n_roots != p->n_roots) return bail(fail(MCE_ERR_ARG, "custom root list must be a permutation of all roots"));
+    // The list holds canonical root ids; check it is a permutation.
+    std::vector<uint8_t> seen(p->n_roots, 0);
+    for (uint64_t i = 0; i < opt.n_roots; ++i) {
+      if (opt.roots[i] >= p->n_roots || seen[opt.roots[i]]) return bail(fail(MCE_ERR_ARG, "root list is not a permutation"));
+      seen[opt.roots[i]] = 1;
+    }
+    PTRY(dalloc(&p->d_perm, p->n_roots));
+    PTRY(hipMemcpy(p->d_perm, opt.roots, p->n_roots * 8, hipMemcpyHostToDevice));
+  }
+
+  const int wpc = opt.waves_per_cu > 0 ? opt.waves_per_cu : 16;
+  p->waves = (uint32_t)std::max<uint64_t>(1, (uint64_t)p->cus * wpc);
+  size_t free_b = 0, total_b = 0;
+  PTRY(hipMemGetInfo(&free_b, &total_b));
+  uint64_t want = opt.slot_bytes ? opt.slot_bytes : worst_need(g.max_degree, max_out);
+  uint64_t cap = std::min<uint64_t>(64ull << 20, (uint64_t)(free_b * 0.45) / p->waves);
+  if (!opt.slot_bytes) want = std::min(want, cap);
+  p->slot_bytes = align_up(std::max<uint64_t>(want, 4096), 256);
+  PTRY(dalloc(&p->d_slots, p->slot_bytes * p->waves));
+
+  PTRY(dalloc(&p->d_ctr, kNumCtr));
+  PTRY(dalloc(&p->d_hist, p->hist_len));
+  if (opt.want_root_stats) {
+    PTRY(dalloc(&p->d_root_nodes, p->n_roots));
+    PTRY(dalloc(&p->d_root_cycles, p->n_roots));
+  }
+  if (opt.want_list) {
+    p->list_cap = opt.list_capacity ? opt.list_capacity : (1ull << 24);
+    PTRY(dalloc(&p->d_list, p->list_cap));
+  }
+  p->deferred_cap = std::max<uint64_t>(1, std::min<uint64_t>(p->n_roots, 1ull << 24));
+  PTRY(dalloc(&p->d_deferred, p->deferred_cap));
+  PTRY(hipStreamSynchronize(p->stream));
+  *out = p;
+  return MCE_OK;
+#undef PTRY
+}
+
+mce_status plan_run_impl(mce_plan *p, mce_result **out) {
+  auto t0 = std::chrono::steady_clock::now();
+  HIP_TRY(hipSetDevice(p->device));
+  auto res = new mce_result();
+  std::memset(res, 0, sizeof(*res));
+  auto bail = [&](mce_status s) {
+    mce_result_free(res);
+    return s;
+  };
+#define RTRY(expr)                                                                          \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return bail(fail(MCE_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+  } while (0)
+  hipStream_t s = p->stream;
+  RTRY(hipMemsetAsync(p->d_ctr, 0, kNumCtr * 8, s));
+  RTRY(hipMemsetAsync(p->d_hist, 0, p->hist_len * 8ull, s));
+  if (p->d_root_nodes) {
+    RTRY(hipMemsetAsync(p->d_root_nodes, 0, p->n_roots * 8, s));
+    RTRY(hipMemsetAsync(p->d_root_cycles, 0, p->n_roots * 8, s));
+  }
+  KArgs a{};
+  a.rowoff = p->d_rowoff;
+  a.col = p->d_col;
+  a.label_hash = p->d_lhash;
+  a.labels = p->d_labels;
+  a.n = p->n;
+  a.roots = p->d_roots;
+  a.n_roots_canon = p->n_roots;
+  a.perm = p->d_perm;
+  a.n_dispatch = p->n_roots;
+  a.reverse = p->d_perm ? 0 : 1;
+  a.root_mode = p->opt.root_mode;
+  a.count = p->d_ctr + kCount;
+  a.hash = p->d_ctr + kHash;
+  a.total_nodes = p->d_ctr + kNodes;
+  a.next_root = p->d_ctr + kNext;
+  a.list_cursor = p->d_ctr + kListCursor;
+  a.n_deferred = p->d_ctr + kNDeferred;
+  a.deferred_need = p->d_ctr + kDeferNeed;
+  a.err = (unsigned int *)(p->d_ctr + kErr);
+  a.hist = p->d_hist;
+  a.hist_len = p->hist_len;
+  a.root_nodes = p->d_root_nodes;
+  a.root_cycles = p->d_root_cycles;
+  a.list = p->d_list;
+  a.list_cap = p->list_cap;
+  a.slots = p->d_slots;
+  a.slot_bytes = p->slot_bytes;
+  a.deferred = p->d_deferred;
+  a.deferred_cap = p->deferred_cap;
+
+  float kernel_ms = 0;
+  res->passes = 0;
+  res->waves = (int32_t)p->waves;
+  if (p->n_roots) {
+    RTRY(hipEventRecord(p->ev[0], s));
+    RTRY(launch_mce(a, p->waves, s));
+    RTRY(hipEventRecord(p->ev[1], s));
+    RTRY(hipEventSynchronize(p->ev[1]));
+    float ms = 0;
+    RTRY(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
+    kernel_ms += ms;
+    res->passes = 1;
+    unsigned long long ctr[kNumCtr];
+    RTRY(hipMemcpy(ctr, p->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+    uint64_t nd = ctr[kNDeferred];
+    if (nd > p->deferred_cap) return bail(fail(MCE_ERR_NOMEM, "deferred-root list overflow"));
+    if (nd) {
+      // Second pass: the deferred roots with slots sized to the largest need.
+      res->deferred_roots = nd;
+      uint64_t slot2 = align_up(std::max<uint64_t>(ctr[kDeferNeed], 4096), 256);
+      size_t free_b = 0, total_b = 0;
+      RTRY(hipMemGetInfo(&free_b, &total_b));
+      uint64_t fit = (uint64_t)(free_b * 0.8) / slot2;
+      uint64_t waves2 = std::min<uint64_t>({(uint64_t)p->waves, nd, fit});
+      if (waves2 == 0)
+        return bail(fail(MCE_ERR_NOMEM, "a root needs " + std::to_string(slot2) + " bytes of scratch"));
+      if (p->slots2_bytes < slot2 * waves2) {
+        if (p->d_slots2) hipFree(p->d_slots2);
+        p->d_slots2 = nullptr;
+        p->slots2_bytes = 0;
+        RTRY(dalloc(&p->d_slots2, slot2 * waves2));
+        p->slots2_bytes = slot2 * waves2;
+      }
+      if (!p->d_deferred2) RTRY(dalloc(&p->d_deferred2, p->deferred_cap));
+      RTRY(hipMemcpyAsync(p->d_deferred2, p->d_deferred, nd * 8, hipMemcpyDeviceToDevice, s));
+      unsigned long long zero3[2] = {0, 0};
+      RTRY(hipMemcpyAsync(p->d_ctr + kNext, zero3, 8, hipMemcpyHostToDevice, s));
+      RTRY(hipMemcpyAsync(p->d_ctr + kNDeferred, zero3, 16, hipMemcpyHostToDevice, s));
+      KArgs b = a;
+      b.perm = p->d_deferred2;
+      b.n_dispatch = nd;
+      b.reverse = 0;
+      b.slots = p->d_slots2;
+      b.slot_bytes = slot2;
+      RTRY(hipEventRecord(p->ev[2], s));
+      RTRY(launch_mce(b, (uint32_t)waves2, s));
+      RTRY(hipEventRecord(p->ev[3], s));
+      RTRY(hipEventSynchronize(p->ev[3]));
+      RTRY(hipEventElapsedTime(&ms, p->ev[2], p->ev[3]));
+      kernel_ms += ms;
+      res->passes = 2;
+      RTRY(hipMemcpy(ctr, p->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+      if (ctr[kNDeferred]) return bail(fail(MCE_ERR_INTERNAL, "root deferred twice"));
+    }
+    uint32_t err = (uint32_t)ctr[kErr];
+    if (err & kErrPTooLarge) return bail(fail(MCE_ERR_UNSUPPORTED, "a root has more than 4096 candidates"));
+    if (err & kErrHistOverflow) return bail(fail(MCE_ERR_INTERNAL, "clique size beyond histogram"));
+    if (err & kErrListFull) return bail(fail(MCE_ERR_NOMEM, "clique list capacity exceeded"));
+    if (err & kErrDeferFull) return bail(fail(MCE_ERR_NOMEM, "deferred-root list overflow"));
+    res->count = ctr[kCount];
+    res->hash = ctr[kHash];
+    res->total_nodes = ctr[kNodes];
+    if (p->d_list) {
+      res->list_len = ctr[kListCursor];
+      res->list = (uint32_t *)std::malloc(std::max<uint64_t>(res->list_len, 1) * 4);
+      if (res->list_len) RTRY(hipMemcpy(res->list, p->d_list, res->list_len * 4, hipMemcpyDeviceToHost));
+    }
+  }
+  res->hist_len = p->hist_len;
+  res->hist = (uint64_t *)std::calloc(p->hist_len, 8);
+  RTRY(hipMemcpy(res->hist, p->d_hist, p->hist_len * 8ull, hipMemcpyDeviceToHost));
+  res->n_roots = p->n_roots;
+  if (p->d_root_nodes) {
+    res->root_nodes = (uint64_t *)std::malloc(std::max<uint64_t>(p->n_roots, 1) * 8);
+    res->root_cycles = (uint64_t *)std::malloc(std::max<uint64_t>(p->n_roots, 1) * 8);
+    if (p->n_roots) {
+      RTRY(hipMemcpy(res->root_nodes, p->d_root_nodes, p->n_roots * 8, hipMemcpyDeviceToHost));
+      RTRY(hipMemcpy(res->root_cycles, p->d_root_cycles, p->n_roots * 8, hipMemcpyDeviceToHost));
+    }
+  }
+  res->kernel_ms = kernel_ms;
+  res->total_ms = ms_since(t0);
+  *out = res;
+  return MCE_OK;
+#undef RTRY
+}
+}  // namespace
+
+extern "C" {
+
+mce_status mce_plan_create(const mce_graph *g, const mce_options *opt, mce_plan **out) {
+  return guarded([&] {
+    if (!g || !out) return fail(MCE_ERR_ARG, "null argument");
+    mce_options o;
+    mce_options_init(&o);
+    if (opt) o = *opt;
+    return plan_create_impl(g->g, o, out);
+  });
+}
+
+mce_status mce_plan_run(mce_plan *p, mce_result **out) {
+  return guarded([&] {
+    if (!p || !out) return fail(MCE_ERR_ARG, "null argument");
+    return plan_run_impl(p, out);
+  });
+}
+
+void mce_plan_free(mce_plan *p) { plan_release(p); }
+
+mce_status mce_plan_roots(const mce_graph *g, int32_t root_mode, uint64_t *out, uint64_t cap, uint64_t *count) {
+  return guarded([&] {
+    if (!g || !count) return fail(MCE_ERR_ARG, "null argument");
+    std::vector<uint64_t> roots;
+    uint64_t max_out;
+    build_roots(g->g, root_mode == MCE_ROOT_VERTEX ? MCE_ROOT_VERTEX : MCE_ROOT_EDGE, roots, max_out);
+    *count = roots.size();
+    if (out) {
+      if (cap < roots.size()) return fail(MCE_ERR_ARG, "root buffer too small");
+      std::memcpy(out, roots.data(), roots.size() * 8);
+    }
+    return MCE_OK;
+  });
+}
+
+mce_status mce_solve(const mce_graph *g, const mce_options *opt, mce_result **out) {
+  return guarded([&] {
+    if (!g || !out) return fail(MCE_ERR_ARG, "null argument");
+    auto t0 = std::chrono::steady_clock::now();
+    mce_plan *p = nullptr;
+    mce_status s = mce_plan_create(g, opt, &p);
+    if (s) return s;
+    s = mce_plan_run(p, out);
+    mce_plan_free(p);
+    if (s == MCE_OK) (*out)->total_ms = ms_since(t0);
+    return s;
+  });
+}
+
+mce_status mce_solve_csr(uint64_t n, const uint64_t *rowoff, const uint32_t *col, const uint32_t *labels,
+                         uint64_t degeneracy, const mce_options *opt, mce_result **out) {
+  mce_graph *g = nullptr;
+  mce_status s = mce_graph_from_csr(n, rowoff, col, labels, degeneracy, &g);
+  if (s) return s;
+  s = mce_solve(g, opt, out);
+  mce_graph_free(g);
+  return s;
+}
+
+void mce_result_free(mce_result *r) {
+  if (!r) return;
+  std::free(r->hist);
+  std::free(r->list);
+  std::free(r->root_nodes);
+  std::free(r->root_cycles);
+  delete r;
+}
+
+}  // extern "C"

@@ -1,0 +1,150 @@
+"""Solver front end, mirroring the reference's operator surface:
+
+* ``BkSolverWrapper(graph, device_id) -> int``  (RDMCE/src/kernels/
+  BkPivotBitBalance.cuh:611-678): upload, enumerate, free; returns the count.
+* ``MceGpuSolver(graph, device_id).Solve()``    (RDMCE/src/mce_gpu.cu:9-22):
+  the same, timed, printing ``MC numbers:`` and ``MCE GPU Time:`` exactly as
+  the reference does (scripts/analyze.sh:97 reads field 4 of the time line).
+
+Beyond the reference (which only counts) ``solve()`` returns the size
+histogram, the clique hash, the optional clique list and per-root search
+statistics.  All work runs in librdmce.so's HIP kernels; there is no CPU path.
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+from ctypes import POINTER, byref, c_uint64, c_void_p
+
+import numpy as np
+
+from . import _lib
+from ._lib import MceOptions, MceResult, check, lib
+from .graph import Graph
+
+
+@dataclasses.dataclass
+class MceOutput:
+    count: int
+    hash: int
+    hist: np.ndarray            # hist[k] = maximal cliques of size k
+    cliques: list | None        # sorted tuples of original labels (want_list)
+    root_nodes: np.ndarray | None
+    root_cycles: np.ndarray | None
+    total_nodes: int
+    deferred_roots: int
+    shared_tasks: int
+    kernel_ms: float
+    total_ms: float
+    passes: int
+    waves: int
+
+    def hist_dict(self) -> dict:
+        return {int(k): int(v) for k, v in enumerate(self.hist) if v}
+
+
+def make_options(device: int = 0, root_mode: int = _lib.ROOT_EDGE, waves_per_cu: int = 0, share_bound: int = 0,
+                 want_list: bool = False, want_root_stats: bool = False, list_capacity: int = 0,
+                 slot_bytes: int = 0, root_order=None, lds_tier: int = 1) -> tuple[MceOptions, object]:
+    o = MceOptions()
+    lib().mce_options_init(byref(o))
+    o.device = device
+    o.root_mode = root_mode
+    o.waves_per_cu = waves_per_cu
+    o.share_bound = share_bound
+    o.want_list = int(want_list)
+    o.want_root_stats = int(want_root_stats)
+    o.list_capacity = list_capacity
+    o.slot_bytes = slot_bytes
+    o.lds_tier = lds_tier
+    keep = None
+    if root_order is not None:
+        keep = np.ascontiguousarray(root_order, dtype=np.uint64)
+        o.schedule = _lib.SCHED_CUSTOM
+        o.roots = keep.ctypes.data_as(POINTER(c_uint64))
+        o.n_roots = keep.size
+    return o, keep
+
+
+def _convert(r: MceResult, want_list: bool, want_root_stats: bool) -> MceOutput:
+    hist = np.ctypeslib.as_array(r.hist, shape=(r.hist_len,)).copy() if r.hist_len else np.zeros(0, np.uint64)
+    cliques = None
+    if want_list:
+        flat = np.ctypeslib.as_array(r.list, shape=(max(r.list_len, 1),))[:r.list_len].copy()
+        cliques = []
+        i = 0
+        while i < flat.size:
+            k = int(flat[i])
+            cliques.append(tuple(sorted(int(x) for x in flat[i + 1:i + 1 + k])))
+            i += k + 1
+    rn = rc = None
+    if want_root_stats and r.n_roots:
+        rn = np.ctypeslib.as_array(r.root_nodes, shape=(r.n_roots,)).copy()
+        rc = np.ctypeslib.as_array(r.root_cycles, shape=(r.n_roots,)).copy()
+    return MceOutput(int(r.count), int(r.hash), hist, cliques, rn, rc, int(r.total_nodes), int(r.deferred_roots),
+                     int(r.shared_tasks), float(r.kernel_ms), float(r.total_ms), int(r.passes), int(r.waves))
+
+
+def solve(graph: Graph, **kw) -> MceOutput:
+    """One full call (upload + enumerate + free), like BkSolverWrapper."""
+    o, keep = make_options(**kw)
+    res = POINTER(MceResult)()
+    check(lib().mce_solve(graph.handle, byref(o), byref(res)))
+    try:
+        return _convert(res.contents, bool(o.want_list), bool(o.want_root_stats))
+    finally:
+        lib().mce_result_free(res)
+        del keep
+
+
+class Plan:
+    """Graph, roots and scratch resident in HBM; ``run()`` enumerates only."""
+
+    def __init__(self, graph: Graph, **kw):
+        self._o, self._keep = make_options(**kw)
+        self._h = c_void_p()
+        check(lib().mce_plan_create(graph.handle, byref(self._o), byref(self._h)))
+
+    def run(self) -> MceOutput:
+        res = POINTER(MceResult)()
+        check(lib().mce_plan_run(self._h, byref(res)))
+        try:
+            return _convert(res.contents, bool(self._o.want_list), bool(self._o.want_root_stats))
+        finally:
+            lib().mce_result_free(res)
+
+    def free(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mce_plan_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.free()
+
+
+def BkSolverWrapper(graph: Graph, device_id: int = 0) -> int:
+    """RDMCE/src/kernels/BkPivotBitBalance.cuh:611 - returns the clique count."""
+    return solve(graph, device=device_id).count
+
+
+class MceGpuSolver:
+    """RDMCE/src/mce_gpu.cu:9-22."""
+
+    def __init__(self, g: Graph, device_id: int = 0):
+        self.g_ = g
+        self.device_id_ = device_id
+        self.mce_count_ = 0
+        self.result: MceOutput | None = None
+
+    def Solve(self) -> int:
+        t0 = time.perf_counter()
+        self.result = solve(self.g_, device=self.device_id_)
+        elapsed = time.perf_counter() - t0
+        self.mce_count_ = self.result.count
+        print(f"\nMC numbers: {self.mce_count_}")
+        print(f"MCE GPU Time: {elapsed}")
+        return self.mce_count_
+
+
+def device_count() -> int:
+    return int(lib().mce_device_count())

@@ -1,0 +1,140 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle, the
+committed goldens and closed forms.  Integer work: every comparison is
+bit-exact (count, histogram, clique hash, clique list, per-root node counts).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import rdmce_amd as R
+from graphs import closed_forms, random_graph
+from oracle_lib import OracleGraph, clique_hash
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def banner(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def oracle_for(g: R.Graph, root_mode=2, root_nodes=False, list_cap=0):
+    og = OracleGraph.from_csr(g.GetRowOffset(), g.GetColIdx(), g.GetLabels(), g.GetDegeneracy())
+    return og.mce(root_mode=root_mode, root_nodes=root_nodes, list_cap=list_cap)
+
+
+def check_same(out, ref, *, lists=False, nodes=False):
+    assert out.count == ref.count
+    assert out.hist_dict() == ref.hist
+    assert out.hash == ref.hash
+    assert sum(out.hist_dict().values()) == out.count
+    if lists:
+        assert sorted(out.cliques) == sorted(tuple(sorted(c)) for c in ref.cliques)
+        assert clique_hash(out.cliques) == out.hash
+    if nodes:
+        assert out.total_nodes == ref.total_nodes
+        assert np.array_equal(out.root_nodes, ref.root_nodes)
+
+
+def test_zachary_full():
+    banner("zachary edge roots")
+    g = R.Graph()
+    g.LoadFromFile(os.path.join(GOLDEN, "zachary.txt"))
+    g.SortByOrder(R.OrderType.DEG)
+    out = R.solve(g, want_list=True, want_root_stats=True)
+    ref = oracle_for(g, root_nodes=True, list_cap=1 << 16)
+    assert out.count == 36
+    assert out.hist_dict() == {2: 11, 3: 21, 4: 2, 5: 2}
+    check_same(out, ref, lists=True, nodes=True)
+
+
+@pytest.mark.parametrize("name,edges,hist", closed_forms(), ids=[c[0] for c in closed_forms()])
+def test_closed_forms(name, edges, hist):
+    banner(f"closed form {name}")
+    g = R.Graph.from_edges(edges)
+    g.SortByOrder(R.OrderType.DEG)
+    out = R.solve(g, want_root_stats=True)
+    assert out.hist_dict() == hist
+    if sum(hist.values()) < 100000:
+        check_same(out, oracle_for(g, root_nodes=True), nodes=True)
+
+
+@pytest.mark.parametrize("n,p,seed", [(30, 0.3, 1), (60, 0.5, 2), (120, 0.7, 3), (300, 0.4, 4), (300, 0.05, 5),
+                                      (80, 0.85, 6), (150, 0.6, 7), (400, 0.3, 8)])
+def test_random_graphs(n, p, seed):
+    banner(f"random n={n} p={p} seed={seed}")
+    g = R.Graph.from_edges(random_graph(n, p, seed))
+    g.SortByOrder(R.OrderType.DEG)
+    out = R.solve(g, want_root_stats=True, want_list=(n <= 120))
+    ref = oracle_for(g, root_nodes=True, list_cap=(1 << 22) if n <= 120 else 0)
+    check_same(out, ref, lists=(n <= 120), nodes=True)
+
+
+def test_er_config1():
+    banner("ER G(2048, 0.008) seed 1")
+    g = R.Graph.erdos_renyi(2048, 0.008, 1)
+    g.SortByOrder(R.OrderType.DEG)
+    out = R.solve(g, want_list=True, want_root_stats=True)
+    ref = oracle_for(g, root_nodes=True, list_cap=1 << 20)
+    check_same(out, ref, lists=True, nodes=True)
+
+
+@pytest.mark.parametrize("order", [R.OrderType.UNCHANGED, R.OrderType.ASC, R.OrderType.DES])
+def test_order_invariance(order):
+    banner(f"order invariance {int(order)}")
+    edges = random_graph(150, 0.5, 11)
+    g0 = R.Graph.from_edges(edges)
+    g0.SortByOrder(R.OrderType.DEG)
+    base = R.solve(g0)
+    g = R.Graph.from_edges(edges)
+    g.SortByOrder(order)
+    out = R.solve(g)
+    assert (out.count, out.hash, out.hist_dict()) == (base.count, base.hash, base.hist_dict())
+
+
+def test_vertex_roots():
+    banner("vertex roots")
+    g = R.Graph.from_edges(random_graph(180, 0.6, 12))
+    g.SortByOrder(R.OrderType.DEG)
+    e = R.solve(g)
+    v = R.solve(g, root_mode=R.ROOT_VERTEX, want_root_stats=True)
+    ref = oracle_for(g, root_mode=1, root_nodes=True)
+    check_same(v, ref, nodes=True)
+    assert (v.count, v.hash, v.hist_dict()) == (e.count, e.hash, e.hist_dict())
+
+
+def test_deferred_pass():
+    banner("deferred second pass")
+    g = R.Graph.from_edges(random_graph(250, 0.45, 13))
+    g.SortByOrder(R.OrderType.DEG)
+    base = R.solve(g, want_root_stats=True)
+    small = R.solve(g, slot_bytes=16384, want_root_stats=True)
+    assert small.passes == 2 and small.deferred_roots > 0
+    assert (small.count, small.hash, small.hist_dict()) == (base.count, base.hash, base.hist_dict())
+    assert np.array_equal(small.root_nodes, base.root_nodes)
+
+
+def test_custom_schedule():
+    banner("custom root permutation")
+    g = R.Graph.from_edges(random_graph(160, 0.55, 14))
+    g.SortByOrder(R.OrderType.DEG)
+    base = R.solve(g, want_root_stats=True)
+    perm = np.random.default_rng(0).permutation(g.GetNumEdges())
+    out = R.solve(g, root_order=perm, want_root_stats=True)
+    assert (out.count, out.hash, out.hist_dict()) == (base.count, base.hash, base.hist_dict())
+    assert np.array_equal(out.root_nodes, base.root_nodes)
+
+
+def test_plan_rerun():
+    banner("plan rerun")
+    g = R.Graph.from_edges(random_graph(140, 0.6, 15))
+    g.SortByOrder(R.OrderType.DEG)
+    plan = R.Plan(g)
+    a = plan.run()
+    b = plan.run()
+    plan.free()
+    assert (a.count, a.hash, a.hist_dict()) == (b.count, b.hash, b.hist_dict())
