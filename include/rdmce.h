@@ -45,7 +45,9 @@ typedef int32_t mce_status;
 
 /* Root schedules (dispatch order of roots; never changes results). */
 #define MCE_SCHED_PLAIN 0    /* reverse edge-slot order, BkPivotBitBalance.cuh:429-431 */
-#define MCE_SCHED_CUSTOM 1   /* caller-supplied root list (mce_options.roots) */
+#define MCE_SCHED_CUSTOM 1   /* caller-supplied list of distinct canonical root ids
+                                (mce_options.roots): all roots in any order, or a
+                                subset - results then cover the listed roots only */
 
 const char *mce_last_error(void);
 const char *mce_version(void);
@@ -115,8 +117,9 @@ typedef struct mce_options {
   int32_t lds_tier;          /* 1 = stage small subproblems in LDS (default), 0 = HBM only */
   uint64_t list_capacity;    /* u32 entries reserved for the clique list; 0 = 1<<24 */
   uint64_t slot_bytes;       /* per-wave HBM scratch; 0 = sized automatically */
-  const uint64_t *roots;     /* MCE_SCHED_CUSTOM: root descriptors in dispatch order */
-  uint64_t n_roots;          /*   (see mce_plan_roots for their encoding) */
+  const uint64_t *roots;     /* MCE_SCHED_CUSTOM: canonical root ids (indices into the
+                                list of mce_plan_roots) in dispatch order */
+  uint64_t n_roots;
   uint64_t reserved[8];
 } mce_options;
 
@@ -139,7 +142,10 @@ typedef struct mce_result {
   double total_ms;           /* wall time of the call */
   int32_t passes;
   int32_t waves;             /* wavefronts launched */
-  uint64_t reserved[8];
+  uint64_t search_bytes;     /* bitmap bytes read by the search (8 per 64-bit word) */
+  uint64_t build_bytes;      /* bytes moved building subproblems: both root adjacency
+                                lists, the candidate list and the bitmap */
+  uint64_t reserved[6];
 } mce_result;
 
 /* BkSolverWrapper(Graph&, device_id), BkPivotBitBalance.cuh:611-678 and
@@ -162,6 +168,11 @@ mce_status mce_plan_roots(const mce_graph *g, int32_t root_mode, uint64_t *out, 
 void mce_plan_free(mce_plan *p);
 
 int32_t mce_device_count(void);
+mce_status mce_device_sync(int32_t device);
+/* Measurement helper (no reference counterpart): times `iters` launches of
+ * a streaming copy kernel over `bytes` (read `bytes`, write `bytes`), used to
+ * calibrate hardware traffic counters against a known byte count. */
+mce_status mce_copy_probe(int32_t device, uint64_t bytes, int32_t iters, double *ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -247,6 +247,7 @@ typedef struct {
   const orc_graph *g;
   int mode;
   uint64_t *outoff; /* mode 2: prefix of out-degrees, n + 1 */
+  const uint64_t *ids; /* explicit canonical root ids, or NULL for [root_begin, root_end) */
   uint64_t root_begin, root_end;
   _Atomic uint64_t next;
   int want_nodes;
@@ -541,7 +542,7 @@ static void solve_root(thr_t *t, uint64_t r) {
     solve_sub(t, p, x);
   }
   t->nodes += t->root_nodes;
-  if (t->sh->want_nodes) t->sh->root_nodes[r - t->sh->root_begin] = t->root_nodes;
+  if (t->sh->want_nodes && !t->sh->ids) t->sh->root_nodes[r - t->sh->root_begin] = t->root_nodes;
 }
 
 static void *worker(void *arg) {
@@ -551,7 +552,7 @@ static void *worker(void *arg) {
     uint64_t b = atomic_fetch_add(&sh->next, 64);
     if (b >= sh->root_end) break;
     uint64_t e = b + 64 < sh->root_end ? b + 64 : sh->root_end;
-    for (uint64_t r = b; r < e; ++r) solve_root(t, r);
+    for (uint64_t r = b; r < e; ++r) solve_root(t, sh->ids ? sh->ids[r] : r);
   }
   return NULL;
 }
@@ -561,14 +562,24 @@ uint64_t orc_num_roots(const orc_graph *g, int root_mode) {
   return g->m;
 }
 
+static int mce_impl(const orc_graph *g, int root_mode, int threads, int want_root_nodes, uint64_t list_cap,
+                    uint64_t root_begin, uint64_t root_end, const uint64_t *ids, orc_result **out);
+
 int orc_mce_range(const orc_graph *g, int root_mode, int threads, int want_root_nodes, uint64_t list_cap,
                   uint64_t root_begin, uint64_t root_end, orc_result **out) {
+  return mce_impl(g, root_mode, threads, want_root_nodes, list_cap, root_begin, root_end, NULL, out);
+}
+
+static int mce_impl(const orc_graph *g, int root_mode, int threads, int want_root_nodes, uint64_t list_cap,
+                    uint64_t root_begin, uint64_t root_end, const uint64_t *ids, orc_result **out) {
   double t0 = now_s();
   shared_t sh;
   memset(&sh, 0, sizeof(sh));
   sh.g = g;
   sh.mode = root_mode == 1 ? 1 : 2;
-  uint64_t nroots = orc_num_roots(g, sh.mode);
+  uint64_t nroots = ids ? root_end : orc_num_roots(g, sh.mode);
+  sh.ids = ids;
+  if (ids) want_root_nodes = 0;
   if (root_end > nroots) root_end = nroots;
   if (root_begin > root_end) root_begin = root_end;
   sh.root_begin = root_begin;
@@ -630,6 +641,14 @@ int orc_mce_range(const orc_graph *g, int root_mode, int threads, int want_root_
   res->seconds = now_s() - t0;
   *out = res;
   return 0;
+}
+
+int orc_mce_roots(const orc_graph *g, int root_mode, int threads, const uint64_t *ids, uint64_t n_ids,
+                  orc_result **out) {
+  uint64_t nroots = orc_num_roots(g, root_mode == 1 ? 1 : 2);
+  for (uint64_t i = 0; i < n_ids; ++i)
+    if (ids[i] >= nroots) return 1;
+  return mce_impl(g, root_mode, threads, 0, 0, 0, n_ids, ids, out);
 }
 
 int orc_mce(const orc_graph *g, int root_mode, int threads, int want_root_nodes, uint64_t list_cap,

@@ -81,6 +81,9 @@ int orc_mce(const orc_graph *g, int root_mode, int threads, int want_root_nodes,
 /* Same, restricted to canonical roots [root_begin, root_end) (timing samples). */
 int orc_mce_range(const orc_graph *g, int root_mode, int threads, int want_root_nodes, uint64_t list_cap,
                   uint64_t root_begin, uint64_t root_end, orc_result **out);
+/* Same over an explicit list of canonical root ids (timing samples). */
+int orc_mce_roots(const orc_graph *g, int root_mode, int threads, const uint64_t *ids, uint64_t n_ids,
+                  orc_result **out);
 /* Textbook list-based Tomita pivoting (max |P ^ N(u)|), vertex-rooted in
  * degeneracy order; count + histogram + hash only.  Independent cross-check. */
 int orc_mce_tomita(const orc_graph *g, orc_result **out);

@@ -63,7 +63,9 @@ class MceResult(Structure):
         ("total_ms", c_double),
         ("passes", c_int32),
         ("waves", c_int32),
-        ("reserved", c_uint64 * 8),
+        ("search_bytes", c_uint64),
+        ("build_bytes", c_uint64),
+        ("reserved", c_uint64 * 6),
     ]
 
 
@@ -124,6 +126,8 @@ def lib() -> ctypes.CDLL:
         "mce_plan_roots": ([G, c_int32, POINTER(c_uint64), c_uint64, POINTER(c_uint64)], c_int32),
         "mce_plan_free": ([c_void_p], None),
         "mce_device_count": ([], c_int32),
+        "mce_device_sync": ([c_int32], c_int32),
+        "mce_copy_probe": ([c_int32, c_uint64, c_int32, POINTER(c_double)], c_int32),
     }
     for name, (args, ret) in sigs.items():
         fn = getattr(L, name)

@@ -295,6 +295,41 @@ void mce_options_init(mce_options *opt) {
   opt->lds_tier = 1;
 }
 
+mce_status mce_copy_probe(int32_t device, uint64_t bytes, int32_t iters, double *ms_per_copy) {
+  if (bytes < 16 || iters <= 0 || !ms_per_copy) return fail(MCE_ERR_ARG, "bad copy probe arguments");
+  HIP_TRY(hipSetDevice(device));
+  void *src = nullptr, *dst = nullptr;
+  HIP_TRY(hipMalloc(&src, bytes));
+  if (hipMalloc(&dst, bytes) != hipSuccess) {
+    (void)hipFree(src);
+    return fail(MCE_ERR_NOMEM, "copy probe allocation failed");
+  }
+  hipEvent_t e0, e1;
+  hipError_t e = hipMemset(src, 1, bytes);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_copy_probe(src, dst, bytes, nullptr);
+  if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+  for (int i = 0; i < iters && e == hipSuccess; ++i) e = launch_copy_probe(src, dst, bytes, nullptr);
+  if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  if (e != hipSuccess) return fail(MCE_ERR_DEVICE, hipGetErrorString(e));
+  *ms_per_copy = ms / iters;
+  return MCE_OK;
+}
+
+mce_status mce_device_sync(int32_t device) {
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipDeviceSynchronize());
+  return MCE_OK;
+}
+
 int32_t mce_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -312,6 +347,7 @@ struct mce_plan {
   int cus = 0;
   uint64_t n = 0;
   uint64_t n_roots = 0;
+  uint64_t n_dispatch = 0;
   uint32_t hist_len = 0;
   uint32_t waves = 0;
   uint64_t slot_bytes = 0;
@@ -325,6 +361,11 @@ struct mce_plan {
   uint64_t *d_roots = nullptr;
   uint64_t *d_perm = nullptr;
   uint8_t *d_slots = nullptr;
+  uint8_t *d_stacks = nullptr;
+  uint64_t stack_bytes = 0;
+  ShareRec *d_recs = nullptr;
+  unsigned int *d_users = nullptr;
+  unsigned long long *d_has_work = nullptr;
   unsigned long long *d_ctr = nullptr;  // see Ctr
   unsigned long long *d_hist = nullptr;
   unsigned long long *d_root_nodes = nullptr;
@@ -335,10 +376,13 @@ struct mce_plan {
   uint64_t *d_deferred2 = nullptr;
   uint8_t *d_slots2 = nullptr;
   uint64_t slots2_bytes = 0;
+  uint8_t *d_stacks2 = nullptr;
+  uint64_t stacks2_bytes = 0;
 };
 
 namespace {
-enum Ctr { kCount, kHash, kNodes, kNext, kListCursor, kNDeferred, kDeferNeed, kErr, kNumCtr };
+enum Ctr { kCount, kHash, kNodes, kNext, kListCursor, kNDeferred, kDeferNeed, kDeferStackNeed, kShared, kActive,
+           kIdle, kErr, kWords, kBuild, kNumCtr };
 
 template <class T>
 hipError_t dalloc(T **p, uint64_t count) {
@@ -350,7 +394,8 @@ void plan_release(mce_plan *p) {
   hipSetDevice(p->device);
   void *ptrs[] = {p->d_rowoff, p->d_col,       p->d_lhash,       p->d_labels,   p->d_roots,    p->d_perm,
                   p->d_slots,  p->d_ctr,       p->d_hist,        p->d_root_nodes, p->d_root_cycles,
-                  p->d_list,   p->d_deferred,  p->d_deferred2,   p->d_slots2};
+                  p->d_list,   p->d_deferred,  p->d_deferred2,   p->d_slots2,   p->d_stacks,
+                  p->d_stacks2, p->d_recs,      p->d_users,       p->d_has_work};
   for (void *q : ptrs)
     if (q) hipFree(q);
   for (auto &e : p->ev)
@@ -361,14 +406,18 @@ void plan_release(mce_plan *p) {
 
 uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
-// Scratch need bound of the largest possible root (see need_bound in the
-// kernel): c <= max degree, p <= max out-degree.
-uint64_t worst_need(uint64_t maxdeg, uint64_t maxout) {
-  uint64_t p = std::min(maxdeg, maxout), c = maxdeg, x = maxdeg;
+// Scratch of the largest possible root: candidates c <= max degree,
+// |P| <= max out-degree.  Data region: candidate list + bitmap; stack:
+// (|P| + 1) level records.
+uint64_t worst_data(uint64_t maxdeg, uint64_t maxout) {
+  uint64_t p = std::min(maxdeg, maxout), c = maxdeg;
+  uint64_t W = (p + 63) / 64;
+  return align_up(4 * c, 256) + align_up(8 * W * align_up(c, 64), 256);
+}
+uint64_t worst_stack(uint64_t maxdeg, uint64_t maxout) {
+  uint64_t p = std::min(maxdeg, maxout), x = maxdeg;
   uint64_t W = (p + 63) / 64, Wx = (x + 63) / 64;
-  uint64_t bm_off = align_up(4 * c, 256);
-  uint64_t st_off = bm_off + align_up(8 * W * align_up(p + x, 64), 256);
-  return st_off + (p + 1) * (4 * W + Wx + 4) * 8;
+  return (p + 1) * (4 * W + Wx + 4) * 8;
 }
 
 mce_status plan_create_impl(const HostGraph &g, const mce_options &opt, mce_plan **out) {
@@ -403,6 +452,7 @@ mce_status plan_create_impl(const HostGraph &g, const mce_options &opt, mce_plan
   build_roots(g, mode, roots, max_out);
   p->n = g.n;
   p->n_roots = roots.size();
+  p->n_dispatch = p->n_roots;
   p->hist_len = (uint32_t)std::min<uint64_t>(max_out + 3, 1ull << 20);
   if (p->hist_len < kLdsHist) p->hist_len = kLdsHist;
 
@@ -423,26 +473,37 @@ mce_status plan_create_impl(const HostGraph &g, const mce_options &opt, mce_plan
   if (p->n_roots) PTRY(hipMemcpy(p->d_roots, roots.data(), p->n_roots * 8, hipMemcpyHostToDevice));
   if (opt.schedule == MCE_SCHED_CUSTOM) {
     if (!opt.roots) return bail(fail(MCE_ERR_ARG, "custom schedule without a root list"));
-    if (opt.n_roots != p->n_roots) return bail(fail(MCE_ERR_ARG, "custom root list must be a permutation of all roots"));
-    // The list holds canonical root ids; check it is a permutation.
+    if (opt.n_roots > p->n_roots) return bail(fail(MCE_ERR_ARG, "custom root list longer than the root set"));
+    // The list holds distinct canonical root ids (all of them, or a subset:
+    // then the results cover exactly the cliques of the listed roots).
     std::vector<uint8_t> seen(p->n_roots, 0);
     for (uint64_t i = 0; i < opt.n_roots; ++i) {
-      if (opt.roots[i] >= p->n_roots || seen[opt.roots[i]]) return bail(fail(MCE_ERR_ARG, "root list is not a permutation"));
+      if (opt.roots[i] >= p->n_roots || seen[opt.roots[i]])
+        return bail(fail(MCE_ERR_ARG, "root list has an out-of-range or repeated id"));
       seen[opt.roots[i]] = 1;
     }
-    PTRY(dalloc(&p->d_perm, p->n_roots));
-    PTRY(hipMemcpy(p->d_perm, opt.roots, p->n_roots * 8, hipMemcpyHostToDevice));
+    p->n_dispatch = opt.n_roots;
+    PTRY(dalloc(&p->d_perm, opt.n_roots));
+    if (opt.n_roots) PTRY(hipMemcpy(p->d_perm, opt.roots, opt.n_roots * 8, hipMemcpyHostToDevice));
   }
 
   const int wpc = opt.waves_per_cu > 0 ? opt.waves_per_cu : 16;
-  p->waves = (uint32_t)std::max<uint64_t>(1, (uint64_t)p->cus * wpc);
+  p->waves = (uint32_t)std::min<uint64_t>(16384, std::max<uint64_t>(1, (uint64_t)p->cus * wpc));
   size_t free_b = 0, total_b = 0;
   PTRY(hipMemGetInfo(&free_b, &total_b));
-  uint64_t want = opt.slot_bytes ? opt.slot_bytes : worst_need(g.max_degree, max_out);
-  uint64_t cap = std::min<uint64_t>(64ull << 20, (uint64_t)(free_b * 0.45) / p->waves);
-  if (!opt.slot_bytes) want = std::min(want, cap);
-  p->slot_bytes = align_up(std::max<uint64_t>(want, 4096), 256);
+  // Each wave gets a data region and a stack; both are sized for the largest
+  // root when that fits in ~45% of free HBM, else capped (larger roots go to
+  // the deferred pass).  slot_bytes, when given, is split evenly.
+  uint64_t cap = std::min<uint64_t>(32ull << 20, (uint64_t)(free_b * 0.22) / p->waves);
+  uint64_t want_d = opt.slot_bytes ? opt.slot_bytes / 2 : std::min(worst_data(g.max_degree, max_out), cap);
+  uint64_t want_s = opt.slot_bytes ? opt.slot_bytes / 2 : std::min(worst_stack(g.max_degree, max_out), cap);
+  p->slot_bytes = align_up(std::max<uint64_t>(want_d, 2048), 256);
+  p->stack_bytes = align_up(std::max<uint64_t>(want_s, 2048), 256);
   PTRY(dalloc(&p->d_slots, p->slot_bytes * p->waves));
+  PTRY(dalloc(&p->d_stacks, p->stack_bytes * p->waves));
+  PTRY(dalloc(&p->d_recs, p->waves));
+  PTRY(dalloc(&p->d_users, p->waves));
+  PTRY(dalloc(&p->d_has_work, (p->waves + 63) / 64));
 
   PTRY(dalloc(&p->d_ctr, kNumCtr));
   PTRY(dalloc(&p->d_hist, p->hist_len));
@@ -477,7 +538,14 @@ mce_status plan_run_impl(mce_plan *p, mce_result **out) {
     if (e_ != hipSuccess) return bail(fail(MCE_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); \
   } while (0)
   hipStream_t s = p->stream;
+  auto reset_sharing = [&]() -> hipError_t {
+    hipError_t e = hipMemsetAsync(p->d_recs, 0, sizeof(ShareRec) * p->waves, s);
+    if (e == hipSuccess) e = hipMemsetAsync(p->d_users, 0, 4ull * p->waves, s);
+    if (e == hipSuccess) e = hipMemsetAsync(p->d_has_work, 0, 8ull * ((p->waves + 63) / 64), s);
+    return e;
+  };
   RTRY(hipMemsetAsync(p->d_ctr, 0, kNumCtr * 8, s));
+  RTRY(reset_sharing());
   RTRY(hipMemsetAsync(p->d_hist, 0, p->hist_len * 8ull, s));
   if (p->d_root_nodes) {
     RTRY(hipMemsetAsync(p->d_root_nodes, 0, p->n_roots * 8, s));
@@ -492,17 +560,30 @@ mce_status plan_run_impl(mce_plan *p, mce_result **out) {
   a.roots = p->d_roots;
   a.n_roots_canon = p->n_roots;
   a.perm = p->d_perm;
-  a.n_dispatch = p->n_roots;
+  a.n_dispatch = p->n_dispatch;
   a.reverse = p->d_perm ? 0 : 1;
   a.root_mode = p->opt.root_mode;
   a.count = p->d_ctr + kCount;
   a.hash = p->d_ctr + kHash;
   a.total_nodes = p->d_ctr + kNodes;
+  a.search_words = p->d_ctr + kWords;
+  a.build_bytes = p->d_ctr + kBuild;
   a.next_root = p->d_ctr + kNext;
   a.list_cursor = p->d_ctr + kListCursor;
   a.n_deferred = p->d_ctr + kNDeferred;
   a.deferred_need = p->d_ctr + kDeferNeed;
+  a.deferred_stack_need = p->d_ctr + kDeferStackNeed;
+  a.shared_tasks = p->d_ctr + kShared;
+  a.active = (unsigned int *)(p->d_ctr + kActive);
+  a.idle = (unsigned int *)(p->d_ctr + kIdle);
   a.err = (unsigned int *)(p->d_ctr + kErr);
+  a.stacks = p->d_stacks;
+  a.stack_bytes = p->stack_bytes;
+  a.recs = p->d_recs;
+  a.users = p->d_users;
+  a.has_work = p->d_has_work;
+  a.share_bound = p->opt.share_bound == 0 ? 24 : p->opt.share_bound;
+  a.n_waves = p->waves;
   a.hist = p->d_hist;
   a.hist_len = p->hist_len;
   a.root_nodes = p->d_root_nodes;
@@ -517,7 +598,7 @@ mce_status plan_run_impl(mce_plan *p, mce_result **out) {
   float kernel_ms = 0;
   res->passes = 0;
   res->waves = (int32_t)p->waves;
-  if (p->n_roots) {
+  if (p->n_dispatch) {
     RTRY(hipEventRecord(p->ev[0], s));
     RTRY(launch_mce(a, p->waves, s));
     RTRY(hipEventRecord(p->ev[1], s));
@@ -533,10 +614,11 @@ mce_status plan_run_impl(mce_plan *p, mce_result **out) {
     if (nd) {
       // Second pass: the deferred roots with slots sized to the largest need.
       res->deferred_roots = nd;
-      uint64_t slot2 = align_up(std::max<uint64_t>(ctr[kDeferNeed], 4096), 256);
+      uint64_t slot2 = align_up(std::max<uint64_t>(ctr[kDeferNeed], 2048), 256);
+      uint64_t stack2 = align_up(std::max<uint64_t>(ctr[kDeferStackNeed], 2048), 256);
       size_t free_b = 0, total_b = 0;
       RTRY(hipMemGetInfo(&free_b, &total_b));
-      uint64_t fit = (uint64_t)(free_b * 0.8) / slot2;
+      uint64_t fit = (uint64_t)(free_b * 0.8) / (slot2 + stack2);
       uint64_t waves2 = std::min<uint64_t>({(uint64_t)p->waves, nd, fit});
       if (waves2 == 0)
         return bail(fail(MCE_ERR_NOMEM, "a root needs " + std::to_string(slot2) + " bytes of scratch"));
@@ -547,17 +629,28 @@ mce_status plan_run_impl(mce_plan *p, mce_result **out) {
         RTRY(dalloc(&p->d_slots2, slot2 * waves2));
         p->slots2_bytes = slot2 * waves2;
       }
+      if (p->stacks2_bytes < stack2 * waves2) {
+        if (p->d_stacks2) hipFree(p->d_stacks2);
+        p->d_stacks2 = nullptr;
+        p->stacks2_bytes = 0;
+        RTRY(dalloc(&p->d_stacks2, stack2 * waves2));
+        p->stacks2_bytes = stack2 * waves2;
+      }
+      RTRY(reset_sharing());
       if (!p->d_deferred2) RTRY(dalloc(&p->d_deferred2, p->deferred_cap));
       RTRY(hipMemcpyAsync(p->d_deferred2, p->d_deferred, nd * 8, hipMemcpyDeviceToDevice, s));
-      unsigned long long zero3[2] = {0, 0};
-      RTRY(hipMemcpyAsync(p->d_ctr + kNext, zero3, 8, hipMemcpyHostToDevice, s));
-      RTRY(hipMemcpyAsync(p->d_ctr + kNDeferred, zero3, 16, hipMemcpyHostToDevice, s));
+      RTRY(hipMemsetAsync(p->d_ctr + kNext, 0, 8, s));
+      RTRY(hipMemsetAsync(p->d_ctr + kNDeferred, 0, 8 * 3, s));
+      RTRY(hipMemsetAsync(p->d_ctr + kActive, 0, 8 * 2, s));
       KArgs b = a;
       b.perm = p->d_deferred2;
       b.n_dispatch = nd;
       b.reverse = 0;
       b.slots = p->d_slots2;
       b.slot_bytes = slot2;
+      b.stacks = p->d_stacks2;
+      b.stack_bytes = stack2;
+      b.n_waves = (uint32_t)waves2;
       RTRY(hipEventRecord(p->ev[2], s));
       RTRY(launch_mce(b, (uint32_t)waves2, s));
       RTRY(hipEventRecord(p->ev[3], s));
@@ -576,6 +669,9 @@ mce_status plan_run_impl(mce_plan *p, mce_result **out) {
     res->count = ctr[kCount];
     res->hash = ctr[kHash];
     res->total_nodes = ctr[kNodes];
+    res->shared_tasks = ctr[kShared];
+    res->search_bytes = ctr[kWords] * 8;
+    res->build_bytes = ctr[kBuild];
     if (p->d_list) {
       res->list_len = ctr[kListCursor];
       res->list = (uint32_t *)std::malloc(std::max<uint64_t>(res->list_len, 1) * 4);

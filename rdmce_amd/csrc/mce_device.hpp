@@ -17,6 +17,26 @@ constexpr uint32_t kErrHistOverflow = 2u;    // clique larger than hist_len - 1
 constexpr uint32_t kErrDeferFull = 4u;       // deferred-root list full
 constexpr uint32_t kErrListFull = 8u;        // clique list capacity exceeded
 
+// One root's subproblem, as every wavefront working on it sees it.
+struct Sub {
+  uint32_t p, xq, W, Wx, rows_pad, LS;
+  uint32_t base_size, v0, v1, bm_owner;
+  uint64_t base_key, canon;
+  const uint32_t *Plist;  // p entries, P0 ascending
+  const uint64_t *bm;     // word-major bitmap
+};
+
+// Exposure record of one wavefront: at most one published level at a time.
+// ctl = epoch << 32 | ntot << 16 | next; a claim is fetch_add(ctl, 1) and
+// succeeds when next < ntot (task index = next).
+struct alignas(128) ShareRec {
+  unsigned long long ctl;
+  unsigned int pending;   // published tasks not yet materialized
+  unsigned int epoch;     // owner's copy of the epoch
+  const uint64_t *snap;   // frozen level record in the owner's stack
+  Sub sub;
+};
+
 struct KArgs {
   // Graph (CSR in HBM).
   const uint64_t *rowoff;
@@ -39,6 +59,8 @@ struct KArgs {
   unsigned long long *hist;     // hist_len entries
   uint32_t hist_len;
   unsigned long long *total_nodes;
+  unsigned long long *search_words;  // bitmap words read by the search
+  unsigned long long *build_bytes;   // bytes moved building subproblems
   unsigned long long *root_nodes;   // per canonical root, nullable
   unsigned long long *root_cycles;  // per canonical root, nullable
   uint32_t *list;                   // nullable
@@ -46,18 +68,31 @@ struct KArgs {
   uint64_t list_cap;
   // Work distribution.
   unsigned long long *next_root;
-  // Per-wave HBM scratch.
+  // Per-wave HBM scratch: data (candidate lists + bitmap) and DFS stack.
   uint8_t *slots;
   uint64_t slot_bytes;
-  // Roots whose scratch need exceeds slot_bytes are deferred to a pass with
-  // larger slots: canonical ids, count, and the largest need seen.
+  uint8_t *stacks;
+  uint64_t stack_bytes;
+  // Sharing (root-down exposure + stealing).
+  ShareRec *recs;                   // one per wave
+  unsigned int *users;              // per wave: other waves reading its data region
+  unsigned long long *has_work;     // bit per wave: record has unclaimed tasks
+  unsigned int *active;             // waves running a root or a task
+  unsigned int *idle;               // waves looking for work
+  unsigned long long *shared_tasks; // tasks taken by a wave other than the publisher
+  int32_t share_bound;              // |P| threshold for exposing a level; <= 0: off
+  uint32_t n_waves;
+  // Roots whose scratch need exceeds the slots are deferred to a pass with
+  // larger slots: canonical ids, count, and the largest needs seen.
   uint64_t *deferred;
   unsigned long long *n_deferred;
   uint64_t deferred_cap;
-  unsigned long long *deferred_need;
+  unsigned long long *deferred_need;        // data region bytes
+  unsigned long long *deferred_stack_need;  // stack bytes
   unsigned int *err;
 };
 
 hipError_t launch_mce(const KArgs &a, uint32_t waves, hipStream_t stream);
+hipError_t launch_copy_probe(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
 
 }  // namespace rdmce

@@ -110,14 +110,37 @@ __device__ __forceinline__ void for_each_p_in_nu(const KArgs &a, uint32_t u, con
   }
 }
 
-struct WaveAcc {
+// Per-wave accumulators (lane 0 flushes them at exit).  `words` counts the
+// 64-bit bitmap words the search reads; `build` the bytes the subproblem
+// build moves (both adjacency lists, the candidate list, the bitmap).
+struct Acc {
   unsigned long long count;
   unsigned long long hash;
+  unsigned long long words;
+  unsigned long long build;
 };
 
-__device__ __forceinline__ void emit(const KArgs &a, WaveAcc &acc, unsigned long long *s_hist, uint32_t size,
-                                     uint64_t key, uint64_t rk, uint32_t W, const uint32_t *Plist, uint32_t v0,
-                                     uint32_t v1, uint32_t base_size) {
+__device__ __forceinline__ unsigned int ld_relaxed(unsigned int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_relaxed(unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned int ld_acquire(unsigned int *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Sleeps about 256 * n cycles (s_sleep takes an immediate operand).
+__device__ __forceinline__ void nap(uint32_t n) {
+  for (uint32_t i = 0; i < n; ++i) __builtin_amdgcn_s_sleep(4);
+}
+__device__ __forceinline__ uint32_t bcast32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t bcast64(uint64_t v) {
+  return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+         __builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
+__device__ __forceinline__ void emit(const KArgs &a, const Sub &s, Acc &acc, unsigned long long *s_hist,
+                                     uint32_t size, uint64_t key, uint64_t rk) {
   if (lane() == 0) {
     acc.count++;
     acc.hash += splitmix64_mix(key);
@@ -128,98 +151,284 @@ __device__ __forceinline__ void emit(const KArgs &a, WaveAcc &acc, unsigned long
       atomicOr(a.err, kErrHistOverflow);
     }
   }
-  if (a.list) {
-    if (lane() == 0) {
-      unsigned long long pos = atomicAdd(a.list_cursor, (unsigned long long)(size + 1));
-      if (pos + size + 1 > a.list_cap) {
-        atomicOr(a.err, kErrListFull);
-      } else {
-        uint32_t *dst = a.list + pos;
-        uint32_t k = 0;
-        dst[k++] = size;
-        dst[k++] = a.labels[v0];
-        if (base_size == 2) dst[k++] = a.labels[v1];
-        for (uint32_t w = 0; w < W; ++w) {
-          uint64_t bits = readlane64(rk, w);
-          while (bits) {
-            uint32_t b = __builtin_ctzll(bits);
-            bits &= bits - 1;
-            dst[k++] = a.labels[Plist[w * 64 + b]];
-          }
+  if (a.list && lane() == 0) {
+    unsigned long long pos = atomicAdd(a.list_cursor, (unsigned long long)(size + 1));
+    if (pos + size + 1 > a.list_cap) {
+      atomicOr(a.err, kErrListFull);
+    } else {
+      uint32_t *dst = a.list + pos;
+      uint32_t k = 0;
+      dst[k++] = size;
+      dst[k++] = a.labels[s.v0];
+      if (s.base_size == 2) dst[k++] = a.labels[s.v1];
+      for (uint32_t w = 0; w < s.W; ++w) {
+        uint64_t bits = readlane64(rk, w);
+        while (bits) {
+          uint32_t b = __builtin_ctzll(bits);
+          bits &= bits - 1;
+          dst[k++] = a.labels[s.Plist[w * 64 + b]];
         }
       }
     }
   }
 }
 
-// Level record layout (u64 words): R[W] P[W] XP[W] V[W] XX[Wx] key meta1 meta2 pad
-//   meta1 = rsize | xxcnt << 32, meta2 = xlo | xhi << 32.
-struct Sub {
-  uint32_t p, xq, W, Wx, rows_pad, LS;
-  const uint32_t *Plist;  // p entries (P0 ascending)
-  uint64_t *bm;
-  uint64_t *stk;
+// Level record layout (u64 words): R[W] P[W] XP[W] V[W] XX[Wx] meta[4]
+//   meta[0] = key (sum of label hashes of the clique so far)
+//   meta[1] = rsize | xxcnt << 32
+//   meta[2] = xlo | xhi << 32   (range of possibly non-zero XX words)
+// Words of R, P, XP and V are held one per lane (lane k = word k).
+struct Lvl {
+  uint64_t rk, pk, xpk, vk;
+  uint64_t key;
+  uint32_t rsize, xxcnt, xlo, xhi;
 };
 
-// Depth-first search of one root.  Returns the number of tree nodes.
-__device__ uint64_t search(const KArgs &a, const Sub &s, WaveAcc &acc, unsigned long long *s_hist, uint64_t base_key,
-                           uint32_t base_size, uint32_t v0, uint32_t v1) {
+__device__ __forceinline__ void store_rpx(uint64_t *rec, const Sub &s, const Lvl &L) {
+  const uint32_t l = lane(), W = s.W;
+  if (l < W) {
+    rec[l] = L.rk;
+    rec[W + l] = L.pk;
+    rec[2 * W + l] = L.xpk;
+  }
+  if (l == 0) {
+    uint64_t *meta = rec + 4 * W + s.Wx;
+    meta[0] = L.key;
+    meta[1] = (uint64_t)L.rsize | ((uint64_t)L.xxcnt << 32);
+    meta[2] = (uint64_t)L.xlo | ((uint64_t)L.xhi << 32);
+  }
+}
+
+__device__ __forceinline__ Lvl load_level(const uint64_t *rec, const Sub &s) {
+  const uint32_t l = lane(), W = s.W;
+  Lvl L;
+  if (l < W) {
+    L.rk = rec[l];
+    L.pk = rec[W + l];
+    L.xpk = rec[2 * W + l];
+    L.vk = rec[3 * W + l];
+  } else {
+    L.rk = L.pk = L.xpk = L.vk = 0;
+  }
+  const uint64_t *meta = rec + 4 * W + s.Wx;
+  L.key = meta[0];
+  uint64_t m1 = meta[1], m2 = meta[2];
+  L.rsize = (uint32_t)m1;
+  L.xxcnt = (uint32_t)(m1 >> 32);
+  L.xlo = (uint32_t)m2;
+  L.xhi = (uint32_t)(m2 >> 32);
+  return L;
+}
+
+// Child of candidate c (a vertex of P): R' = R + c, P' = P & N(c),
+// XP' = XP & N(c), XX' = rows of XX adjacent to c, followed by aggressive
+// node generation: every i in P' adjacent to all of P' \ {i} moves to R'
+// (BkPivotBitBalance.cuh:351-382); XP' and XX' are then restricted to the
+// common neighbours of the moved vertices.  Writes the child's record to
+// `out` and returns it.
+__device__ Lvl make_child(const KArgs &a, const Sub &s, const Lvl &par, const uint64_t *par_xx, uint32_t c,
+                          uint64_t *out, Acc &acc) {
+  const uint32_t l = lane(), W = s.W, rp = s.rows_pad, p = s.p;
+  const bool wl = l < W;
+  const uint64_t *bm = s.bm;
+  const uint32_t kc = c >> 6;
+  const uint64_t cbit = 1ull << (c & 63);
+  uint64_t rowc = wl ? bm[(uint64_t)l * rp + c] : 0;
+  acc.words += W;
+  Lvl ch;
+  ch.rk = par.rk | (l == kc ? cbit : 0);
+  ch.pk = par.pk & rowc;
+  ch.xpk = par.xpk & rowc;
+  ch.vk = 0;
+  ch.key = par.key + a.label_hash[s.Plist[c]];
+  ch.rsize = par.rsize + 1;
+  uint64_t mk = 0;
+  uint32_t psz = wave_sum_u32((uint32_t)__popcll(ch.pk));
+  if (psz > 1) {
+    for (uint32_t j = 0; j < W; ++j) {
+      uint64_t w = readlane64(ch.pk, j);
+      if (w == 0) continue;
+      acc.words += (uint64_t)__popcll(w) * W;
+      bool is = (w >> l) & 1;
+      uint32_t cnt = 0;
+      if (is) {
+        uint32_t i = j * 64 + l;
+        for (uint32_t k = 0; k < W; ++k) cnt += __popcll(readlane64(ch.pk, k) & ~bm[(uint64_t)k * rp + i]);
+      }
+      uint64_t m = __ballot(is && cnt == 1);
+      if (l == j) mk = m;
+    }
+    if (__ballot(mk != 0)) {
+      ch.rk |= mk;
+      ch.pk &= ~mk;
+      for (uint32_t j = 0; j < W; ++j) {
+        uint64_t w = readlane64(mk, j);
+        while (w) {
+          uint32_t bb = (uint32_t)__builtin_ctzll(w);
+          w &= w - 1;
+          uint32_t mm = j * 64 + bb;
+          acc.words += W;
+          if (wl) ch.xpk &= bm[(uint64_t)l * rp + mm];
+          ch.key += a.label_hash[s.Plist[mm]];
+          ch.rsize++;
+        }
+      }
+    }
+  }
+  const bool anym = __ballot(mk != 0) != 0;
+  uint64_t *xx2 = out + 4 * W;
+  const uint32_t cb = c & 63;
+  uint32_t cnt2 = 0, nlo = par.xhi, nhi = par.xlo;
+  for (uint32_t j = par.xlo; j < par.xhi; ++j) {
+    uint64_t w = par_xx[j];
+    uint64_t nw = 0;
+    if (w != 0) {
+      acc.words += (uint64_t)__popcll(w) * (anym ? W + 1 : 1);
+      bool hit = false;
+      if ((w >> l) & 1) {
+        uint32_t u = p + j * 64 + l;
+        hit = (bm[(uint64_t)kc * rp + u] >> cb) & 1;
+        if (hit && anym) {
+          for (uint32_t k = 0; k < W && hit; ++k) hit = (readlane64(mk, k) & ~bm[(uint64_t)k * rp + u]) == 0;
+        }
+      }
+      nw = __ballot(hit);
+    }
+    if (l == 0) xx2[j] = nw;
+    if (nw) {
+      cnt2 += __popcll(nw);
+      nlo = j < nlo ? j : nlo;
+      nhi = j + 1;
+    }
+  }
+  if (cnt2 == 0) nlo = nhi = 0;
+  ch.xxcnt = cnt2;
+  ch.xlo = nlo;
+  ch.xhi = nhi;
+  store_rpx(out, s, ch);
+  wave_fence();
+  return ch;
+}
+
+// Tries to take one task from record r (lane 0 only).  Returns the task
+// index or -1; *last is set when it was the final task of the publication.
+__device__ __forceinline__ int claim_task(ShareRec *r, bool *last) {
+  unsigned long long c = ld_relaxed(&r->ctl);
+  if ((c & 0xFFFF) >= ((c >> 16) & 0xFFFF)) return -1;
+  unsigned long long old = __hip_atomic_fetch_add(&r->ctl, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t next = (uint32_t)(old & 0xFFFF), ntot = (uint32_t)((old >> 16) & 0xFFFF);
+  if (next >= ntot) return -1;
+  *last = next + 1 == ntot;
+  return (int)next;
+}
+
+// Child for task `idx` of a frozen level record: the idx-th lowest remaining
+// candidate t of V, with P and XP as sequential DFS would hold them when it
+// reaches t (candidates of V below t moved from P to XP) - the same child
+// the publisher would have generated (BkPivotBitBalance.cuh:40-70).
+__device__ Lvl materialize(const KArgs &a, const Sub &s, const uint64_t *snap, uint32_t idx, uint64_t *out,
+                           Acc &acc) {
+  const uint32_t l = lane(), W = s.W;
+  const bool wl = l < W;
+  Lvl par = load_level(snap, s);
+  uint32_t cnt = wl ? (uint32_t)__popcll(par.vk) : 0;
+  uint32_t before = 0, jt = 0;
+  for (uint32_t j = 0; j < W; ++j) {
+    uint32_t cj = __builtin_amdgcn_readlane(cnt, j);
+    if (before + cj > idx) {
+      jt = j;
+      break;
+    }
+    before += cj;
+  }
+  uint64_t bits = readlane64(par.vk, jt);
+  for (uint32_t k = before; k < idx; ++k) bits &= bits - 1;
+  uint32_t bt = (uint32_t)__builtin_ctzll(bits);
+  uint64_t below = l < jt ? par.vk : (l == jt ? (par.vk & low_mask(bt)) : 0);
+  par.pk &= ~below;
+  par.xpk |= below;
+  return make_child(a, s, par, snap + 4 * W, jt * 64 + bt, out, acc);
+}
+
+__device__ __forceinline__ void publish_bit(const KArgs &a, uint32_t w, bool set) {
+  if (set) atomicOr(&a.has_work[w >> 6], 1ull << (w & 63));
+  else atomicAnd(&a.has_work[w >> 6], ~(1ull << (w & 63)));
+}
+
+// Depth-first search from the record at level 0 of `stk` (a root or a
+// materialized task).  Exposes levels root-down while other waves are idle
+// (BkPivotBitBalance.cuh:87-138): when a branching node has |P| >=
+// share_bound, the shallowest not yet exposed ancestor level is frozen and
+// its remaining candidates become tasks; the owner then never backtracks
+// into a frozen level but claims its remaining tasks like any other wave.
+// Returns the number of nodes this wave evaluated.
+__device__ uint64_t run_dfs(const KArgs &a, const Sub &s, uint64_t *stk, Acc &acc, unsigned long long *s_hist,
+                            uint32_t me, bool &published) {
   const uint32_t W = s.W, p = s.p, rp = s.rows_pad, LS = s.LS;
   const uint32_t l = lane();
   const bool wl = l < W;
   const uint64_t *bm = s.bm;
+  const bool sharing = a.share_bound > 0;
+  ShareRec *my = a.recs + me;
   uint64_t nodes = 0;
-
-  // Level 0: R = {}, P = P0, XP = {}, XX = X0'.
-  uint64_t rk = 0, pk = wl ? low_mask((uint64_t)p - (uint64_t)l * 64) : 0, xpk = 0, vk = 0;
-  if (wl && (uint64_t)l * 64 >= p) pk = 0;
-  uint64_t key = base_key;
-  uint32_t rsize = 0, xxcnt = s.xq, xlo = 0, xhi = s.Wx;
-  {
-    uint64_t *rec = s.stk;
-    for (uint32_t j = l; j < s.Wx; j += 64) rec[4 * W + j] = low_mask((uint64_t)s.xq - (uint64_t)j * 64);
-    if (wl) {
-      rec[l] = rk;
-      rec[W + l] = pk;
-      rec[2 * W + l] = xpk;
-    }
-    wave_fence();
-  }
-  int level = 0;
-  // state machine: 0 = evaluate node, 1 = next child, 2 = backtrack
-  int st = 0;
+  Lvl L = load_level(stk, s);
+  int level = 0, st = 0;
+  int S = -1, Lstar = -1;  // levels <= S are frozen; Lstar: deepest level worth exposing
+  uint32_t tick = 0;
   for (;;) {
-    uint64_t *rec = s.stk + (uint64_t)level * LS;
-    uint64_t *XX = rec + 4 * W;
+    uint64_t *rec = stk + (uint64_t)level * LS;
+    if (sharing && S < Lstar && S + 1 < level && ((++tick & 31) == 0)) {
+      bool go = false;
+      if (l == 0) go = ld_relaxed(a.idle) > 0 && ld_relaxed(&my->pending) == 0;
+      if (bcast32(go)) {
+        const uint32_t lp = (uint32_t)(S + 1);
+        const uint64_t *src = stk + (uint64_t)lp * LS;
+        uint32_t n = wave_sum_u32(wl ? (uint32_t)__popcll(src[3 * W + l]) : 0u);
+        if (n > 0 && l == 0) {
+          __hip_atomic_store(&my->pending, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          my->snap = src;
+          my->sub = s;
+          uint32_t ep = my->epoch + 1;
+          my->epoch = ep;
+          __hip_atomic_store(&my->ctl, ((unsigned long long)ep << 32) | ((unsigned long long)n << 16),
+                             __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          publish_bit(a, me, true);
+        }
+        if (n > 0) published = true;
+        S = (int)lp;
+      }
+    }
     if (st == 0) {
       ++nodes;
-      uint32_t psz = wave_sum_u32((uint32_t)__popcll(pk));
+      uint32_t psz = wave_sum_u32((uint32_t)__popcll(L.pk));
       if (psz == 0) {
-        if (__ballot(xpk != 0) == 0 && xxcnt == 0)
-          emit(a, acc, s_hist, base_size + rsize, key, rk, W, s.Plist, v0, v1, base_size);
+        if (__ballot(L.xpk != 0) == 0 && L.xxcnt == 0) emit(a, s, acc, s_hist, s.base_size + L.rsize, L.key, L.rk);
         st = 2;
         continue;
       }
       // Pivot: argmin over candidate rows u of (|P \ N(u)|, u).
       uint64_t best = ~0ull;
       for (uint32_t j = 0; j < W; ++j) {
-        uint64_t cw = readlane64(pk | xpk, j);
+        uint64_t cw = readlane64(L.pk | L.xpk, j);
         if (cw == 0) continue;
+        acc.words += (uint64_t)__popcll(cw) * W;
         if ((cw >> l) & 1) {
           uint32_t u = j * 64 + l;
           uint32_t cnt = 0;
-          for (uint32_t k = 0; k < W; ++k) cnt += __popcll(readlane64(pk, k) & ~bm[(uint64_t)k * rp + u]);
+          for (uint32_t k = 0; k < W; ++k) cnt += __popcll(readlane64(L.pk, k) & ~bm[(uint64_t)k * rp + u]);
           uint64_t kk = ((uint64_t)cnt << 32) | u;
           best = kk < best ? kk : best;
         }
       }
-      for (uint32_t j = xlo; j < xhi; ++j) {
+      const uint64_t *XX = rec + 4 * W;
+      for (uint32_t j = L.xlo; j < L.xhi; ++j) {
         uint64_t cw = XX[j];
         if (cw == 0) continue;
+        acc.words += (uint64_t)__popcll(cw) * W;
         if ((cw >> l) & 1) {
           uint32_t u = p + j * 64 + l;
           uint32_t cnt = 0;
-          for (uint32_t k = 0; k < W; ++k) cnt += __popcll(readlane64(pk, k) & ~bm[(uint64_t)k * rp + u]);
+          for (uint32_t k = 0; k < W; ++k) cnt += __popcll(readlane64(L.pk, k) & ~bm[(uint64_t)k * rp + u]);
           uint64_t kk = ((uint64_t)cnt << 32) | u;
           best = kk < best ? kk : best;
         }
@@ -229,193 +438,105 @@ __device__ uint64_t search(const KArgs &a, const Sub &s, WaveAcc &acc, unsigned 
         st = 2;
         continue;
       }
-      uint32_t piv = (uint32_t)best;
-      vk = wl ? (pk & ~bm[(uint64_t)l * rp + piv]) : 0;
-      if (wl) rec[3 * W + l] = vk;
+      const uint32_t piv = (uint32_t)best;
+      L.vk = wl ? (L.pk & ~bm[(uint64_t)l * rp + piv]) : 0;
+      acc.words += W;
+      if (wl) rec[3 * W + l] = L.vk;
+      if (sharing && (int)psz >= a.share_bound) Lstar = level;
       st = 1;
       continue;
     }
     if (st == 1) {
-      uint64_t vm = __ballot(vk != 0);
+      uint64_t vm = __ballot(L.vk != 0);
       if (vm == 0) {
         st = 2;
         continue;
       }
-      uint32_t kc = (uint32_t)__builtin_ctzll(vm);
-      uint32_t b = (uint32_t)__builtin_ctzll(readlane64(vk, kc));
-      uint32_t c = kc * 64 + b;
-      uint64_t cbit = 1ull << b;
+      const uint32_t kc = (uint32_t)__builtin_ctzll(vm);
+      const uint32_t b = (uint32_t)__builtin_ctzll(readlane64(L.vk, kc));
+      const uint64_t cbit = 1ull << b;
       if (l == kc) {
-        vk &= ~cbit;
-        pk &= ~cbit;
-        xpk |= cbit;
-        rec[W + l] = pk;
-        rec[2 * W + l] = xpk;
-        rec[3 * W + l] = vk;
+        L.vk &= ~cbit;
+        L.pk &= ~cbit;
+        L.xpk |= cbit;
+        rec[W + l] = L.pk;
+        rec[2 * W + l] = L.xpk;
+        rec[3 * W + l] = L.vk;
       }
-      // Child: R' = R + c, P' = P & N(c), XP' = XP & N(c).
-      uint64_t rowc = wl ? bm[(uint64_t)l * rp + c] : 0;
-      uint64_t r2 = rk | (l == kc ? cbit : 0);
-      uint64_t p2 = pk & rowc;
-      uint64_t x2 = xpk & rowc;
-      uint64_t key2 = key + a.label_hash[s.Plist[c]];
-      uint32_t rs2 = rsize + 1;
-      // Aggressive node generation: every i in P' adjacent to all of P' \ {i}
-      // moves to R' (BkPivotBitBalance.cuh:351-382).
-      uint64_t mk = 0;
-      uint32_t psz2 = wave_sum_u32((uint32_t)__popcll(p2));
-      if (psz2 > 1) {
-        for (uint32_t j = 0; j < W; ++j) {
-          uint64_t w = readlane64(p2, j);
-          if (w == 0) continue;
-          bool is = (w >> l) & 1;
-          uint32_t cnt = 0;
-          if (is) {
-            uint32_t i = j * 64 + l;
-            for (uint32_t k = 0; k < W; ++k) cnt += __popcll(readlane64(p2, k) & ~bm[(uint64_t)k * rp + i]);
-          }
-          uint64_t m = __ballot(is && cnt == 1);
-          if (l == j) mk = m;
-        }
-        if (__ballot(mk != 0)) {
-          r2 |= mk;
-          p2 &= ~mk;
-          for (uint32_t j = 0; j < W; ++j) {
-            uint64_t w = readlane64(mk, j);
-            while (w) {
-              uint32_t bb = (uint32_t)__builtin_ctzll(w);
-              w &= w - 1;
-              uint32_t mm = j * 64 + bb;
-              if (wl) x2 &= bm[(uint64_t)l * rp + mm];
-              key2 += a.label_hash[s.Plist[mm]];
-              rs2++;
-            }
-          }
-        }
-      }
-      bool anym = __ballot(mk != 0) != 0;
-      // XX' = rows x of XX adjacent to c and to every moved vertex.
-      uint64_t *rec2 = rec + LS;
-      uint64_t *XX2 = rec2 + 4 * W;
-      uint32_t cw = c >> 6, cb = c & 63;
-      uint32_t cnt2 = 0, nlo = xhi, nhi = xlo;
-      for (uint32_t j = xlo; j < xhi; ++j) {
-        uint64_t w = XX[j];
-        uint64_t nw = 0;
-        if (w != 0) {
-          bool hit = false;
-          if ((w >> l) & 1) {
-            uint32_t u = p + j * 64 + l;
-            hit = (bm[(uint64_t)cw * rp + u] >> cb) & 1;
-            if (hit && anym) {
-              for (uint32_t k = 0; k < W && hit; ++k) hit = (readlane64(mk, k) & ~bm[(uint64_t)k * rp + u]) == 0;
-            }
-          }
-          nw = __ballot(hit);
-        }
-        if (l == 0) XX2[j] = nw;
-        if (nw) {
-          cnt2 += __popcll(nw);
-          nlo = j < nlo ? j : nlo;
-          nhi = j + 1;
-        }
-      }
-      if (cnt2 == 0) nlo = nhi = 0;
-      if (wl) {
-        rec2[l] = r2;
-        rec2[W + l] = p2;
-        rec2[2 * W + l] = x2;
-      }
-      if (l == 0) {
-        uint64_t *meta = rec2 + 4 * W + s.Wx;
-        meta[0] = key2;
-        meta[1] = (uint64_t)rs2 | ((uint64_t)cnt2 << 32);
-        meta[2] = (uint64_t)nlo | ((uint64_t)nhi << 32);
-      }
-      wave_fence();
+      L = make_child(a, s, L, rec + 4 * W, kc * 64 + b, rec + LS, acc);
       ++level;
-      rk = r2;
-      pk = p2;
-      xpk = x2;
-      vk = 0;
-      key = key2;
-      rsize = rs2;
-      xxcnt = cnt2;
-      xlo = nlo;
-      xhi = nhi;
       st = 0;
       continue;
     }
-    // st == 2: backtrack to the parent level.
+    // st == 2: backtrack.
+    if (S >= 0 && level - 1 <= S) {
+      // The parent level is frozen: serve its remaining tasks.
+      int idx = -1;
+      bool last = false;
+      if (l == 0) idx = claim_task(my, &last);
+      idx = (int)bcast32((uint32_t)idx);
+      if (idx >= 0) {
+        if (bcast32(last) && l == 0) publish_bit(a, me, false);
+        const uint64_t *snap = stk + (uint64_t)S * LS;
+        level = S + 1;
+        L = materialize(a, s, snap, (uint32_t)idx, stk + (uint64_t)level * LS, acc);
+        if (l == 0) __hip_atomic_fetch_sub(&my->pending, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        Lstar = Lstar < S ? Lstar : S;
+        st = 0;
+        continue;
+      }
+      // All tasks taken; wait until the takers have copied their part of the
+      // frozen levels out of this stack.
+      if (l == 0) {
+        uint32_t n = 1;
+        while (ld_acquire(&my->pending) != 0) {
+          nap(n);
+          n = n < 8 ? n * 2 : 8;
+        }
+      }
+      wave_fence();
+      break;
+    }
     if (level == 0) break;
     --level;
-    rec = s.stk + (uint64_t)level * LS;
-    if (wl) {
-      rk = rec[l];
-      pk = rec[W + l];
-      xpk = rec[2 * W + l];
-      vk = rec[3 * W + l];
-    } else {
-      rk = pk = xpk = vk = 0;
-    }
-    if (level == 0) {
-      key = base_key;
-      rsize = 0;
-      xxcnt = s.xq;
-      xlo = 0;
-      xhi = s.Wx;
-    } else {
-      const uint64_t *meta = rec + 4 * W + s.Wx;
-      key = meta[0];
-      uint64_t m1 = meta[1], m2 = meta[2];
-      rsize = (uint32_t)m1;
-      xxcnt = (uint32_t)(m1 >> 32);
-      xlo = (uint32_t)m2;
-      xhi = (uint32_t)(m2 >> 32);
-    }
+    L = load_level(stk + (uint64_t)level * LS, s);
+    Lstar = Lstar < level ? Lstar : level;
     st = 1;
   }
   return nodes;
 }
 
-__device__ void defer_root(const KArgs &a, uint64_t canon, uint64_t need) {
+__device__ void defer_root(const KArgs &a, uint64_t canon, uint64_t data_need, uint64_t stack_need) {
   if (lane() == 0) {
     unsigned long long pos = atomicAdd(a.n_deferred, 1ull);
     if (pos < a.deferred_cap) a.deferred[pos] = canon;
     else atomicOr(a.err, kErrDeferFull);
-    atomicMax(a.deferred_need, (unsigned long long)need);
+    atomicMax(a.deferred_need, (unsigned long long)data_need);
+    atomicMax(a.deferred_stack_need, (unsigned long long)stack_need);
   }
 }
 
-// Scratch need of a root before its bitmap is built (x instead of x', p+1
-// levels): an upper bound on what it needs in any pass.
-__device__ __forceinline__ uint64_t need_bound(uint64_t c, uint64_t p, uint64_t x) {
-  uint64_t W = (p + 63) / 64, Wx = (x + 63) / 64;
-  uint64_t bm_off = align_up(4 * c, 256);
-  uint64_t st_off = bm_off + align_up(8 * W * align_up(p + x, 64), 256);
-  return st_off + (p + 1) * (4 * W + Wx + 4) * 8;
-}
-
-// Builds and searches one root.  Returns tree nodes.
-__device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t canon, WaveAcc &acc,
-                               unsigned long long *s_hist) {
+// Builds root `canon` in this wave's data region and searches it.
+// Returns tree nodes evaluated by this wave.
+__device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk, uint32_t me, uint64_t canon, Acc &acc,
+                               unsigned long long *s_hist, bool &published) {
   const uint32_t l = lane();
   const uint64_t desc = a.roots[canon];
   uint32_t *C = (uint32_t *)slot;
   const uint64_t ccap = a.slot_bytes / 4;
-  uint32_t v0, v1 = 0, base_size;
-  uint64_t base_key;
+  Sub s;
+  s.canon = canon;
+  s.bm_owner = me;
   uint64_t c = 0, x = 0;
-  const uint32_t *Plist;
   if (a.root_mode == 2) {
-    v0 = (uint32_t)(desc >> 32);
-    v1 = (uint32_t)desc;
-    base_size = 2;
-    base_key = a.label_hash[v0] + a.label_hash[v1];
+    s.v0 = (uint32_t)(desc >> 32);
+    s.v1 = (uint32_t)desc;
+    s.base_size = 2;
+    s.base_key = a.label_hash[s.v0] + a.label_hash[s.v1];
     // C = N(v0) ^ N(v1): lanes take 64 entries of the shorter list at a time
     // and binary-search them in the longer one (BkPivotBitBalance.cuh:449-460).
-    const uint32_t *A = a.col + a.rowoff[v0], *B = a.col + a.rowoff[v1];
-    uint64_t na = a.rowoff[v0 + 1] - a.rowoff[v0], nb = a.rowoff[v1 + 1] - a.rowoff[v1];
+    const uint32_t *A = a.col + a.rowoff[s.v0], *B = a.col + a.rowoff[s.v1];
+    uint64_t na = a.rowoff[s.v0 + 1] - a.rowoff[s.v0], nb = a.rowoff[s.v1 + 1] - a.rowoff[s.v1];
     if (na > nb) {
       const uint32_t *t = A;
       A = B;
@@ -439,32 +560,41 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t canon, Wa
         if (at < ccap) C[at] = val;
       }
       c += __popcll(m);
-      x += __popcll(__ballot(found && val < v1));
+      x += __popcll(__ballot(found && val < s.v1));
     }
+    acc.build += 4 * (na + nb);
   } else {
-    v0 = (uint32_t)desc;
-    base_size = 1;
-    base_key = a.label_hash[v0];
-    const uint32_t *N = a.col + a.rowoff[v0];
-    c = a.rowoff[v0 + 1] - a.rowoff[v0];
-    x = lower_bound(N, 0, c, v0);
+    s.v0 = (uint32_t)desc;
+    s.v1 = 0;
+    s.base_size = 1;
+    s.base_key = a.label_hash[s.v0];
+    const uint32_t *N = a.col + a.rowoff[s.v0];
+    c = a.rowoff[s.v0 + 1] - a.rowoff[s.v0];
+    x = lower_bound(N, 0, c, s.v0);
     for (uint64_t i = l; i < c && i < ccap; i += 64) C[i] = N[i];
+    acc.build += 4 * c;
   }
   const uint64_t p = c - x;
   if (p == 0) {
-    if (x == 0) emit(a, acc, s_hist, base_size, base_key, 0, 0, nullptr, v0, v1, base_size);
+    if (x == 0) {
+      s.W = 0;
+      s.Plist = nullptr;
+      emit(a, s, acc, s_hist, s.base_size, s.base_key, 0);
+    }
     return 0;
   }
   if (p > kMaxP) {
     if (l == 0) atomicOr(a.err, kErrPTooLarge);
     return 0;
   }
-  if (c > ccap) {  // candidate list alone exceeds the slot
-    defer_root(a, canon, need_bound(c, p, x));
+  const uint64_t W = (p + 63) / 64;
+  if (c > ccap) {  // candidate list alone exceeds the data region
+    defer_root(a, canon, align_up(4 * c, 256) + align_up(8 * W * align_up(c, 64), 256),
+               (p + 1) * (4 * W + (x + 63) / 64 + 4) * 8);
     return 0;
   }
   wave_fence();
-  Plist = C + x;
+  const uint32_t *Plist = C + x;
   // X0' = X0 rows with at least one neighbour in P0, compacted in place
   // (graph_gpu.cuh:90-106).
   uint64_t xq = 0;
@@ -484,24 +614,20 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t canon, Wa
     xq += __popcll(m);
   }
   wave_fence();
-  const uint32_t W = (uint32_t)((p + 63) / 64);
   const uint32_t Wx = (uint32_t)((xq + 63) / 64);
   const uint64_t rows = p + xq;
   const uint32_t rows_pad = (uint32_t)align_up(rows, 64);
   const uint64_t bm_off = align_up(4 * c, 256);
-  const uint64_t st_off = bm_off + align_up(8ull * W * rows_pad, 256);
-  const uint32_t LS = 4 * W + Wx + 4;
-  if (st_off + (p + 1) * LS * 8 > a.slot_bytes) {
-    // Bitmap too large for the slot, or the stack bound p + 1 levels is:
-    // build it only when the bitmap fits (the exact depth bound follows).
-    if (st_off > a.slot_bytes) {
-      defer_root(a, canon, st_off + (p + 1) * LS * 8);
-      return 0;
-    }
+  const uint64_t data_need = bm_off + align_up(8ull * W * rows_pad, 256);
+  const uint32_t LS = (uint32_t)(4 * W + Wx + 4);
+  if (data_need > a.slot_bytes) {
+    defer_root(a, canon, data_need, (p + 1) * LS * 8);
+    return 0;
   }
   uint64_t *bm = (uint64_t *)(slot + bm_off);
+  acc.build += 4 * c + 8 * W * rows_pad;
   // Bitmap (word-major): row i is P0[i] for i < p, else X0'[i - p].
-  for (uint64_t i = l; i < (uint64_t)W * rows_pad; i += 64) bm[i] = 0;
+  for (uint64_t i = l; i < W * rows_pad; i += 64) bm[i] = 0;
   wave_fence();
   uint32_t maxdeg = 0;
   for (uint64_t i = l; i < rows; i += 64) {
@@ -523,24 +649,95 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t canon, Wa
     if (i < p) maxdeg = deg > maxdeg ? deg : maxdeg;
   }
   maxdeg = wave_max_u32(maxdeg);
-  wave_fence();
+  // Depth bound: a clique inside P0 has at most maxdeg + 1 vertices.
   const uint64_t levels = (p < (uint64_t)maxdeg + 1 ? p : (uint64_t)maxdeg + 1) + 1;
-  const uint64_t need = st_off + levels * LS * 8;
-  if (need > a.slot_bytes) {
-    defer_root(a, canon, need);
+  if (levels * LS * 8 > a.stack_bytes) {
+    defer_root(a, canon, data_need, levels * LS * 8);
     return 0;
   }
-  Sub s;
   s.p = (uint32_t)p;
   s.xq = (uint32_t)xq;
-  s.W = W;
+  s.W = (uint32_t)W;
   s.Wx = Wx;
   s.rows_pad = rows_pad;
   s.LS = LS;
   s.Plist = Plist;
   s.bm = bm;
-  s.stk = (uint64_t *)(slot + st_off);
-  return search(a, s, acc, s_hist, base_key, base_size, v0, v1);
+  // Level 0: R = {}, P = P0, XP = {}, XX = X0'.
+  Lvl L0;
+  L0.rk = 0;
+  L0.pk = l < W ? low_mask(p - (uint64_t)l * 64) : 0;
+  L0.xpk = 0;
+  L0.vk = 0;
+  L0.key = s.base_key;
+  L0.rsize = 0;
+  L0.xxcnt = (uint32_t)xq;
+  L0.xlo = 0;
+  L0.xhi = Wx;
+  for (uint32_t j = l; j < Wx; j += 64) stk[4 * W + j] = low_mask(xq - (uint64_t)j * 64);
+  store_rpx(stk, s, L0);
+  wave_fence();
+  return run_dfs(a, s, stk, acc, s_hist, me, published);
+}
+
+// Looks for a published task (preferring waves on this XCD, which share
+// this wave's L2) and runs it.  Returns true when a task was run.
+__device__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk, Acc &acc, unsigned long long *s_hist,
+                      bool &is_idle) {
+  const uint32_t l = lane();
+  const uint32_t nwords = (a.n_waves + 63) / 64;
+  const uint64_t xcd = 0x0101010101010101ull << (me & 7);
+  int victim = -1;
+  for (uint32_t base = 0; base < nwords && victim < 0; base += 64) {
+    uint32_t j = base + l;
+    uint64_t w = j < nwords ? ld_relaxed(&a.has_work[j]) : 0;
+    if (j == (me >> 6)) w &= ~(1ull << (me & 63));
+    uint64_t near = __ballot((w & xcd) != 0);
+    uint64_t any = __ballot(w != 0);
+    uint64_t pick = near ? near : any;
+    if (pick) {
+      // rotate so that thieves starting from different waves spread out
+      uint32_t start = (me >> 6) & 63;
+      uint64_t rot = (pick >> start) | (start ? pick << (64 - start) : 0);
+      uint32_t jl = ((uint32_t)__builtin_ctzll(rot) + start) & 63;
+      uint64_t word = readlane64(w, jl);
+      if (near) word &= xcd;
+      uint32_t sb = (me + 1) & 63;
+      uint64_t r2 = (word >> sb) | (sb ? word << (64 - sb) : 0);
+      uint32_t bit = ((uint32_t)__builtin_ctzll(r2) + sb) & 63;
+      victim = (int)((base + jl) * 64 + bit);
+    }
+  }
+  if (victim < 0 || (uint32_t)victim >= a.n_waves) return false;
+  ShareRec *r = a.recs + victim;
+  int idx = -1;
+  bool last = false;
+  if (l == 0) idx = claim_task(r, &last);
+  idx = (int)bcast32((uint32_t)idx);
+  if (idx < 0) return false;
+  if (l == 0) {
+    if (last) publish_bit(a, (uint32_t)victim, false);
+    if (is_idle) atomicSub(a.idle, 1u);
+    atomicAdd(a.active, 1u);
+  }
+  is_idle = false;
+  Sub s = r->sub;
+  const uint64_t *snap = r->snap;
+  if (l == 0) atomicAdd(&a.users[s.bm_owner], 1u);
+  materialize(a, s, snap, (uint32_t)idx, stk, acc);
+  if (l == 0) __hip_atomic_fetch_sub(&r->pending, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t t0 = clock64();
+  bool pub_unused = false;
+  uint64_t nodes = run_dfs(a, s, stk, acc, s_hist, me, pub_unused);
+  if (l == 0) {
+    if (a.root_nodes && nodes) atomicAdd(&a.root_nodes[s.canon], (unsigned long long)nodes);
+    if (a.root_cycles) atomicAdd(&a.root_cycles[s.canon], (unsigned long long)(clock64() - t0));
+    atomicAdd(a.total_nodes, (unsigned long long)nodes);
+    atomicAdd(a.shared_tasks, 1ull);
+    __hip_atomic_fetch_sub(&a.users[s.bm_owner], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    atomicSub(a.active, 1u);
+  }
+  return true;
 }
 
 __global__ __launch_bounds__(64) void mce_kernel(KArgs a) {
@@ -548,30 +745,74 @@ __global__ __launch_bounds__(64) void mce_kernel(KArgs a) {
   const uint32_t l = lane();
   for (uint32_t i = l; i < kLdsHist; i += 64) s_hist[i] = 0;
   __syncthreads();
-  const uint64_t wave = blockIdx.x;
-  uint8_t *slot = a.slots + wave * a.slot_bytes;
-  WaveAcc acc{0, 0};
+  const uint32_t me = blockIdx.x;
+  uint8_t *slot = a.slots + (uint64_t)me * a.slot_bytes;
+  uint64_t *stk = (uint64_t *)(a.stacks + (uint64_t)me * a.stack_bytes);
+  Acc acc{0, 0, 0, 0};
   unsigned long long nodes_total = 0;
+  bool roots_left = true, is_idle = false;
+  bool published = false;  // others may be reading this wave's data region
+  uint32_t backoff = 1;
   for (;;) {
-    unsigned long long i = 0;
-    if (l == 0) i = atomicAdd(a.next_root, 1ull);
-    i = ((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(i >> 32)) << 32) |
-        __builtin_amdgcn_readfirstlane((uint32_t)i);
-    if (i >= a.n_dispatch) break;
-    uint64_t canon = a.perm ? a.perm[i] : (a.reverse ? a.n_dispatch - 1 - i : i);
-    uint64_t t0 = clock64();
-    uint64_t nodes = solve_root(a, slot, canon, acc, s_hist);
-    nodes_total += nodes;
-    if (l == 0) {
-      if (a.root_nodes && nodes) atomicAdd(&a.root_nodes[canon], (unsigned long long)nodes);
-      if (a.root_cycles) atomicAdd(&a.root_cycles[canon], (unsigned long long)(clock64() - t0));
+    bool did = false;
+    if (roots_left) {
+      bool free_slot = true;
+      if (published) {
+        if (l == 0) free_slot = ld_relaxed(&a.users[me]) == 0;
+        free_slot = bcast32(free_slot);
+        if (free_slot) {
+          if (l == 0) (void)ld_acquire(&a.users[me]);  // order our rebuild after their reads
+          published = false;
+        }
+      }
+      if (free_slot) {
+        unsigned long long i = 0;
+        if (l == 0) i = atomicAdd(a.next_root, 1ull);
+        i = bcast64(i);
+        if (i < a.n_dispatch) {
+          if (l == 0) {
+            if (is_idle) atomicSub(a.idle, 1u);
+            atomicAdd(a.active, 1u);
+          }
+          is_idle = false;
+          uint64_t canon = a.perm ? a.perm[i] : (a.reverse ? a.n_dispatch - 1 - i : i);
+          uint64_t t0 = clock64();
+          uint64_t nodes = solve_root(a, slot, stk, me, canon, acc, s_hist, published);
+          nodes_total += nodes;
+          if (l == 0) {
+            if (a.root_nodes && nodes) atomicAdd(&a.root_nodes[canon], (unsigned long long)nodes);
+            if (a.root_cycles) atomicAdd(&a.root_cycles[canon], (unsigned long long)(clock64() - t0));
+            atomicSub(a.active, 1u);
+          }
+          did = true;
+          backoff = 1;
+        } else {
+          roots_left = false;
+        }
+      }
     }
+    if (did) continue;
+    if (!is_idle) {
+      if (l == 0) atomicAdd(a.idle, 1u);
+      is_idle = true;
+    }
+    if (a.share_bound > 0 && steal(a, me, stk, acc, s_hist, is_idle)) {
+      backoff = 1;
+      continue;
+    }
+    bool done = false;
+    if (l == 0) done = !roots_left && ld_relaxed(a.active) == 0;
+    if (bcast32(done)) break;
+    if (l == 0) nap(backoff);
+    backoff = backoff < 64 ? backoff * 2 : 64;
   }
   __syncthreads();
   if (l == 0) {
     if (acc.count) atomicAdd(a.count, acc.count);
     if (acc.hash) atomicAdd(a.hash, acc.hash);
     if (nodes_total) atomicAdd(a.total_nodes, nodes_total);
+    if (acc.words) atomicAdd(a.search_words, acc.words);
+    if (acc.build) atomicAdd(a.build_bytes, acc.build);
   }
   for (uint32_t i = l; i < kLdsHist && i < a.hist_len; i += 64)
     if (s_hist[i]) atomicAdd(&a.hist[i], s_hist[i]);
@@ -584,4 +825,23 @@ hipError_t launch_mce(const KArgs &a, uint32_t waves, hipStream_t stream) {
   return hipGetLastError();
 }
 
+}  // namespace rdmce
+
+namespace rdmce {
+namespace {
+// Streaming copy used to calibrate hardware traffic counters (FETCH_SIZE,
+// WRITE_SIZE) against a kernel of known byte count.
+__global__ __launch_bounds__(256) void copy_probe_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                         uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+hipError_t launch_copy_probe(const void *src, void *dst, uint64_t bytes, hipStream_t stream) {
+  uint64_t n = bytes / 16;
+  uint32_t blocks = (uint32_t)std::min<uint64_t>(8192, (n + 255) / 256 + 1);
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(blocks), dim3(256), 0, stream, (const uint4 *)src, (uint4 *)dst, n);
+  return hipGetLastError();
+}
 }  // namespace rdmce

@@ -38,6 +38,8 @@ class MceOutput:
     total_ms: float
     passes: int
     waves: int
+    search_bytes: int
+    build_bytes: int
 
     def hist_dict(self) -> dict:
         return {int(k): int(v) for k, v in enumerate(self.hist) if v}
@@ -82,7 +84,8 @@ def _convert(r: MceResult, want_list: bool, want_root_stats: bool) -> MceOutput:
         rn = np.ctypeslib.as_array(r.root_nodes, shape=(r.n_roots,)).copy()
         rc = np.ctypeslib.as_array(r.root_cycles, shape=(r.n_roots,)).copy()
     return MceOutput(int(r.count), int(r.hash), hist, cliques, rn, rc, int(r.total_nodes), int(r.deferred_roots),
-                     int(r.shared_tasks), float(r.kernel_ms), float(r.total_ms), int(r.passes), int(r.waves))
+                     int(r.shared_tasks), float(r.kernel_ms), float(r.total_ms), int(r.passes), int(r.waves),
+                     int(r.search_bytes), int(r.build_bytes))
 
 
 def solve(graph: Graph, **kw) -> MceOutput:

@@ -47,6 +47,7 @@ def lib():
         L.orc_mce.argtypes = [POINTER(OrcGraph), c_int, c_int, c_int, c_uint64, PR]
         L.orc_mce_range.argtypes = [POINTER(OrcGraph), c_int, c_int, c_int, c_uint64, c_uint64, c_uint64, PR]
         L.orc_mce_tomita.argtypes = [POINTER(OrcGraph), PR]
+        L.orc_mce_roots.argtypes = [POINTER(OrcGraph), c_int, c_int, POINTER(c_uint64), c_uint64, PR]
         L.orc_result_free.argtypes = [POINTER(OrcResult)]
         L.orc_splitmix64.argtypes = [c_uint64]
         L.orc_splitmix64.restype = c_uint64
@@ -148,6 +149,17 @@ class OracleGraph:
                                 byref(r))
         try:
             return _convert(r.contents, root_nodes, list_cap)
+        finally:
+            lib().orc_result_free(r)
+
+    def mce_roots(self, ids, root_mode=2, threads=0):
+        ids = np.ascontiguousarray(ids, dtype=np.uint64)
+        r = POINTER(OrcResult)()
+        if lib().orc_mce_roots(self.p, root_mode, threads, ids.ctypes.data_as(POINTER(c_uint64)), ids.size,
+                               byref(r)) != 0:
+            raise ValueError("root id out of range")
+        try:
+            return _convert(r.contents, False, 0)
         finally:
             lib().orc_result_free(r)
 

@@ -1,0 +1,86 @@
+"""The C-ABI library loads and exports every function include/rdmce.h
+declares; argument errors return status codes (no exit).  CPU only: no
+compute call touches a GPU here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import rdmce_amd as R
+from rdmce_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "rdmce.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mce_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ["mce_solve", "mce_solve_csr", "mce_graph_load_coo_text", "mce_graph_load_bin",
+                 "mce_graph_store_bin", "mce_graph_order", "mce_result_free", "mce_plan_create", "mce_plan_run"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_exports_via_nm():
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (mce_[a-z0-9_]+)", out))
+    assert set(declared_functions()) <= exported
+
+
+def test_cli_builds_and_rejects_bad_usage():
+    cli = os.path.join(ROOT, "rdmce_amd", "rdmce")
+    assert os.path.exists(cli)
+    r = subprocess.run([cli], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    r = subprocess.run([cli, "-i", "x.txt", "-d", "0,1"], capture_output=True, text=True)
+    assert r.returncode == 1
+
+
+def test_cli_convert_only(tmp_path):
+    cli = os.path.join(ROOT, "rdmce_amd", "rdmce")
+    src = tmp_path / "z.txt"
+    src.write_text(open(os.path.join(ROOT, "tests", "golden", "zachary.txt")).read())
+    r = subprocess.run([cli, "-i", str(src), "-c"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert "vertices: 34 \tedges: 78" in r.stdout
+    assert "max degree: 17 \tdegeneracy: 4" in r.stdout
+    g = R.Graph()
+    g.LoadFromBin(str(tmp_path / "z.bin"))
+    assert g.GetNumEdges() == 78
+
+
+def test_argument_errors_are_status_codes():
+    L = _lib.lib()
+    out = ctypes.c_void_p()
+    assert L.mce_graph_load_coo_text(None, ctypes.byref(out)) == _lib.MCE_ERR_ARG
+    assert b"null" in L.mce_last_error()
+    assert L.mce_gen_er(10, 1.5, 0, ctypes.byref(out)) == _lib.MCE_ERR_ARG
+    assert L.mce_gen_rmat(0, 16, 0.5, 0.2, 0.2, 0, ctypes.byref(out)) == _lib.MCE_ERR_ARG
+
+
+def test_solve_without_device_fails_cleanly():
+    if R.device_count() > 0:
+        pytest.skip("a GPU is present")
+    g = R.Graph.from_edges([(0, 1), (1, 2)])
+    with pytest.raises(R.MceError) as e:
+        R.solve(g)
+    assert e.value.status == _lib.MCE_ERR_DEVICE
+
+
+def test_options_struct_roundtrip():
+    o = _lib.MceOptions()
+    _lib.lib().mce_options_init(ctypes.byref(o))
+    assert o.root_mode == _lib.ROOT_EDGE and o.lds_tier == 1 and o.schedule == 0

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import rdmce_amd as R
-from graphs import closed_forms, random_graph
+from graphs import closed_forms, expected_er_cliques, random_graph
 from oracle_lib import OracleGraph, clique_hash
 
 pytestmark = pytest.mark.gpu
@@ -69,9 +69,10 @@ def test_random_graphs(n, p, seed):
     banner(f"random n={n} p={p} seed={seed}")
     g = R.Graph.from_edges(random_graph(n, p, seed))
     g.SortByOrder(R.OrderType.DEG)
-    out = R.solve(g, want_root_stats=True, want_list=(n <= 120))
-    ref = oracle_for(g, root_nodes=True, list_cap=(1 << 22) if n <= 120 else 0)
-    check_same(out, ref, lists=(n <= 120), nodes=True)
+    lists = expected_er_cliques(n, p) < 3e5
+    out = R.solve(g, want_root_stats=True, want_list=lists)
+    ref = oracle_for(g, root_nodes=True, list_cap=(1 << 24) if lists else 0)
+    check_same(out, ref, lists=lists, nodes=True)
 
 
 def test_er_config1():
@@ -112,7 +113,7 @@ def test_deferred_pass():
     g = R.Graph.from_edges(random_graph(250, 0.45, 13))
     g.SortByOrder(R.OrderType.DEG)
     base = R.solve(g, want_root_stats=True)
-    small = R.solve(g, slot_bytes=16384, want_root_stats=True)
+    small = R.solve(g, slot_bytes=4096, want_root_stats=True)
     assert small.passes == 2 and small.deferred_roots > 0
     assert (small.count, small.hash, small.hist_dict()) == (base.count, base.hash, base.hist_dict())
     assert np.array_equal(small.root_nodes, base.root_nodes)
@@ -138,3 +139,31 @@ def test_plan_rerun():
     b = plan.run()
     plan.free()
     assert (a.count, a.hash, a.hist_dict()) == (b.count, b.hash, b.hist_dict())
+
+
+@pytest.mark.parametrize("tau", [1, 2, 4, 24, -1])
+def test_sharing_tree_invariance(tau):
+    """Root-down exposure must not change the tree: counts, hash, histogram
+    and per-root node counts identical for every sharing bound."""
+    banner(f"sharing tau={tau}")
+    g = R.Graph.from_edges(random_graph(300, 0.5, 21))
+    g.SortByOrder(R.OrderType.DEG)
+    out = R.solve(g, share_bound=tau, want_root_stats=True)
+    ref = oracle_for(g, root_nodes=True)
+    check_same(out, ref, nodes=True)
+    if tau in (1, 2):
+        assert out.shared_tasks > 0
+
+
+def test_sharing_single_heavy_root():
+    """One root holds all the work (Moon-Moser K_{3x9} plus a pendant):
+    other waves can only help by stealing."""
+    banner("sharing heavy root")
+    from graphs import moon_moser
+    g = R.Graph.from_edges(moon_moser(9))
+    g.SortByOrder(R.OrderType.DEG)
+    out = R.solve(g, share_bound=2, want_root_stats=True)
+    assert out.hist_dict() == {9: 3 ** 9}
+    ref = oracle_for(g, root_nodes=True)
+    check_same(out, ref, nodes=True)
+    assert out.shared_tasks > 0
