@@ -120,7 +120,12 @@ typedef struct mce_options {
   const uint64_t *roots;     /* MCE_SCHED_CUSTOM: canonical root ids (indices into the
                                 list of mce_plan_roots) in dispatch order */
   uint64_t n_roots;
-  uint64_t reserved[8];
+  int32_t use_graph;         /* 1 = replay the run (resets + enumeration kernel) as a
+                                captured hipGraph */
+  uint32_t lds_bytes;        /* LDS per wave for the LDS tier; 0 = CU LDS / waves_per_cu */
+  const uint8_t *eager;      /* optional, one flag per dispatched root: expose its
+                                top level at once (a pre-launch split) */
+  uint64_t reserved[6];
 } mce_options;
 
 void mce_options_init(mce_options *opt);
@@ -166,6 +171,92 @@ mce_status mce_plan_run(mce_plan *p, mce_result **out);
  * graph's order, sorted ascending; vertex roots: v).  Host copy. */
 mce_status mce_plan_roots(const mce_graph *g, int32_t root_mode, uint64_t *out, uint64_t cap, uint64_t *count);
 void mce_plan_free(mce_plan *p);
+
+/* ------------------------------------------------------------------------ */
+/* Batches: many independent graphs enumerated by one persistent launch      */
+/* (no reference counterpart: the reference solves one graph per launch,     */
+/* BkPivotBitBalance.cuh:661)                                                */
+/* ------------------------------------------------------------------------ */
+typedef struct mce_batch mce_batch;
+typedef struct mce_batch_result {
+  uint64_t n_graphs;
+  uint32_t hist_stride;      /* hist[g * hist_stride + k] = cliques of size k in graph g */
+  uint64_t *count;           /* per graph */
+  uint64_t *hash;            /* per graph */
+  uint64_t *hist;
+  uint64_t total_count;
+  uint64_t total_nodes;
+  double kernel_ms;          /* device time of the launch (graph replay when captured) */
+  double total_ms;
+  int32_t graph_replay;      /* 1 when the run was a hipGraph replay */
+  int32_t passes;
+  uint64_t reserved[6];
+} mce_batch_result;
+
+/* Concatenates `count` graphs (each in the order it should be searched in)
+ * into one device-resident plan. */
+mce_status mce_batch_create(const mce_graph *const *graphs, uint64_t count, const mce_options *opt,
+                            mce_batch **out);
+mce_status mce_batch_run(mce_batch *b, mce_batch_result **out);
+void mce_batch_result_free(mce_batch_result *r);
+void mce_batch_free(mce_batch *b);
+
+/* ------------------------------------------------------------------------ */
+/* Autotuner (replaces the reference's compile-time WARP_PER_BLOCK,          */
+/* BLOCK_PER_SM and TASK_SHARE_BOUND variants, RDMCE/CMakeLists.txt:47-65,    */
+/* scripts/compile.sh:53-80)                                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct mce_tune_result {
+  int32_t waves_per_cu;
+  int32_t share_bound;
+  int32_t lds_tier;
+  int32_t trials;
+  double best_ms;            /* kernel time of the winning setting on the sample */
+  double base_ms;            /* kernel time of the default setting on the sample */
+  uint64_t reserved[4];
+} mce_tune_result;
+
+/* Times the plan's kernels on a deterministic sample of its roots (every
+ * `stride`-th dispatched root; 1 = all) for each setting of a small grid
+ * (waves per CU x LDS tier x sharing bound), checks that every setting
+ * gives the same clique count and hash, and keeps the fastest for later
+ * mce_plan_run calls. */
+mce_status mce_plan_autotune(mce_plan *p, uint64_t stride, mce_tune_result *out);
+mce_status mce_batch_autotune(mce_batch *b, uint64_t stride, mce_tune_result *out);
+
+/* ------------------------------------------------------------------------ */
+/* Learned cost model (no reference counterpart; the nearest reference      */
+/* behaviour is its fixed dispatch of high-core roots first,                 */
+/* BkPivotBitBalance.cuh:429-431).  An FP32 MLP (8 -> 32 -> 32 -> 1, ReLU)   */
+/* predicting log(1 + cost) of each edge root; trained and evaluated by the   */
+/* library's HIP kernels.  Used only to order roots and flag eager splits,  */
+/* so it never changes a result.                                             */
+/* ------------------------------------------------------------------------ */
+#define MCE_NFEAT 8
+typedef struct mce_cost_model mce_cost_model;
+
+/* Per canonical edge root (order of mce_plan_roots), MCE_NFEAT floats:
+ * log1p(deg v0), log1p(deg v1), core v0 / degeneracy, core v1 / degeneracy,
+ * log1p(outdeg v0), log1p(outdeg v1), log1p(t), log1p(t)^2 with
+ * t = |N+(v1) & N(v0)| (the size of the root's candidate set).  Computed on
+ * the GPU.  out == NULL only reports the root count. */
+mce_status mce_root_features(mce_graph *g, int32_t device, float *out, uint64_t cap, uint64_t *n_roots);
+uint32_t mce_cost_model_num_params(void);     /* trainable + 2 * MCE_NFEAT normalisation */
+uint32_t mce_cost_model_num_trainable(void);
+mce_status mce_cost_model_create(uint64_t seed, mce_cost_model **out);
+mce_status mce_cost_model_get_params(const mce_cost_model *m, float *out, uint64_t cap);
+mce_status mce_cost_model_set_params(mce_cost_model *m, const float *in, uint64_t n);
+/* Sets the input standardisation (per-feature mean and 1/std) from X. */
+mce_status mce_cost_model_fit_normalizer(mce_cost_model *m, const float *X, uint64_t n);
+/* Mean squared error over n samples and its gradient w.r.t. the trainable
+ * parameters (deterministic reduction order). */
+mce_status mce_cost_model_gradient(const mce_cost_model *m, int32_t device, const float *X, const float *Y,
+                                   uint64_t n, float *grad, float *loss);
+/* Minibatch Adam; returns the last epoch's mean squared error. */
+mce_status mce_cost_model_train(mce_cost_model *m, int32_t device, const float *X, const float *Y, uint64_t n,
+                                uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *final_loss);
+mce_status mce_cost_model_predict(const mce_cost_model *m, int32_t device, const float *X, uint64_t n, float *out);
+void mce_cost_model_free(mce_cost_model *m);
 
 int32_t mce_device_count(void);
 mce_status mce_device_sync(int32_t device);

@@ -41,7 +41,39 @@ class MceOptions(Structure):
         ("slot_bytes", c_uint64),
         ("roots", POINTER(c_uint64)),
         ("n_roots", c_uint64),
-        ("reserved", c_uint64 * 8),
+        ("use_graph", c_int32),
+        ("lds_bytes", c_uint32),
+        ("eager", POINTER(ctypes.c_uint8)),
+        ("reserved", c_uint64 * 6),
+    ]
+
+
+class MceBatchResult(Structure):
+    _fields_ = [
+        ("n_graphs", c_uint64),
+        ("hist_stride", c_uint32),
+        ("count", POINTER(c_uint64)),
+        ("hash", POINTER(c_uint64)),
+        ("hist", POINTER(c_uint64)),
+        ("total_count", c_uint64),
+        ("total_nodes", c_uint64),
+        ("kernel_ms", c_double),
+        ("total_ms", c_double),
+        ("graph_replay", c_int32),
+        ("passes", c_int32),
+        ("reserved", c_uint64 * 6),
+    ]
+
+
+class MceTuneResult(Structure):
+    _fields_ = [
+        ("waves_per_cu", c_int32),
+        ("share_bound", c_int32),
+        ("lds_tier", c_int32),
+        ("trials", c_int32),
+        ("best_ms", c_double),
+        ("base_ms", c_double),
+        ("reserved", c_uint64 * 4),
     ]
 
 
@@ -127,6 +159,12 @@ def lib() -> ctypes.CDLL:
         "mce_plan_free": ([c_void_p], None),
         "mce_device_count": ([], c_int32),
         "mce_device_sync": ([c_int32], c_int32),
+        "mce_batch_create": ([POINTER(c_void_p), c_uint64, POINTER(MceOptions), PG], c_int32),
+        "mce_batch_run": ([c_void_p, POINTER(POINTER(MceBatchResult))], c_int32),
+        "mce_batch_result_free": ([POINTER(MceBatchResult)], None),
+        "mce_batch_free": ([c_void_p], None),
+        "mce_plan_autotune": ([c_void_p, c_uint64, POINTER(MceTuneResult)], c_int32),
+        "mce_batch_autotune": ([c_void_p, c_uint64, POINTER(MceTuneResult)], c_int32),
         "mce_copy_probe": ([c_int32, c_uint64, c_int32, POINTER(c_double)], c_int32),
     }
     for name, (args, ret) in sigs.items():

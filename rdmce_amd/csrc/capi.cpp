@@ -10,22 +10,21 @@
 #include <string>
 #include <vector>
 
-#include "../../include/rdmce.h"
+#include "capi_internal.hpp"
 #include "generators.hpp"
-#include "host_graph.hpp"
 #include "mce_device.hpp"
 #include "parallel.hpp"
 #include "rng.hpp"
 
 using namespace rdmce;
 
-struct mce_graph {
-  HostGraph g;
-};
-
 namespace {
 thread_local std::string g_err;
+}  // namespace
 
+void rdmce::set_last_error(const std::string &msg) { g_err = msg; }
+
+namespace {
 mce_status fail(mce_status s, const std::string &msg) {
   g_err = msg;
   return s;
@@ -339,7 +338,8 @@ int32_t mce_device_count(void) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Plan
+// Plan: one graph, or a batch of disjoint graphs concatenated, resident in
+// HBM with its roots and per-wave scratch.
 // ---------------------------------------------------------------------------
 struct mce_plan {
   mce_options opt{};
@@ -349,8 +349,17 @@ struct mce_plan {
   uint64_t n_roots = 0;
   uint64_t n_dispatch = 0;
   uint32_t hist_len = 0;
+  // launch configuration (the autotuner may change it between runs)
+  int waves_per_cu = 16;
   uint32_t waves = 0;
+  uint32_t max_waves = 0;
+  uint32_t lds_bytes = 0;
+  uint32_t lds_per_cu = 160u << 10;
+  uint32_t lds_per_block_max = 64u << 10;
+  int share_bound = 24;
+  int lds_tier = 1;
   uint64_t slot_bytes = 0;
+  uint64_t stack_bytes = 0;
   uint64_t deferred_cap = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -360,9 +369,9 @@ struct mce_plan {
   uint32_t *d_labels = nullptr;
   uint64_t *d_roots = nullptr;
   uint64_t *d_perm = nullptr;
+  uint8_t *d_eager = nullptr;
   uint8_t *d_slots = nullptr;
   uint8_t *d_stacks = nullptr;
-  uint64_t stack_bytes = 0;
   ShareRec *d_recs = nullptr;
   unsigned int *d_users = nullptr;
   unsigned long long *d_has_work = nullptr;
@@ -378,6 +387,20 @@ struct mce_plan {
   uint64_t slots2_bytes = 0;
   uint8_t *d_stacks2 = nullptr;
   uint64_t stacks2_bytes = 0;
+  // batches
+  uint64_t n_graphs = 0;
+  uint32_t g_hist_stride = 0;
+  uint32_t *d_root_graph = nullptr;
+  unsigned long long *d_gcount = nullptr;
+  unsigned long long *d_ghash = nullptr;
+  unsigned long long *d_ghist = nullptr;
+  // captured run (hipGraph)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+};
+
+struct mce_batch {
+  mce_plan *plan = nullptr;
 };
 
 namespace {
@@ -389,30 +412,39 @@ hipError_t dalloc(T **p, uint64_t count) {
   return hipMalloc((void **)p, std::max<uint64_t>(count, 1) * sizeof(T));
 }
 
+void drop_graph(mce_plan *p) {
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->graph_exec = nullptr;
+  p->graph = nullptr;
+}
+
 void plan_release(mce_plan *p) {
   if (!p) return;
-  hipSetDevice(p->device);
-  void *ptrs[] = {p->d_rowoff, p->d_col,       p->d_lhash,       p->d_labels,   p->d_roots,    p->d_perm,
-                  p->d_slots,  p->d_ctr,       p->d_hist,        p->d_root_nodes, p->d_root_cycles,
-                  p->d_list,   p->d_deferred,  p->d_deferred2,   p->d_slots2,   p->d_stacks,
-                  p->d_stacks2, p->d_recs,      p->d_users,       p->d_has_work};
+  (void)hipSetDevice(p->device);
+  drop_graph(p);
+  void *ptrs[] = {p->d_rowoff,   p->d_col,         p->d_lhash,      p->d_labels,      p->d_roots,
+                  p->d_perm,     p->d_eager,       p->d_slots,      p->d_ctr,         p->d_hist,
+                  p->d_root_nodes, p->d_root_cycles, p->d_list,     p->d_deferred,    p->d_deferred2,
+                  p->d_slots2,   p->d_stacks,      p->d_stacks2,    p->d_recs,        p->d_users,
+                  p->d_has_work, p->d_root_graph,  p->d_gcount,     p->d_ghash,       p->d_ghist};
   for (void *q : ptrs)
-    if (q) hipFree(q);
+    if (q) (void)hipFree(q);
   for (auto &e : p->ev)
-    if (e) hipEventDestroy(e);
-  if (p->stream) hipStreamDestroy(p->stream);
+    if (e) (void)hipEventDestroy(e);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
 
 uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 // Scratch of the largest possible root: candidates c <= max degree,
-// |P| <= max out-degree.  Data region: candidate list + bitmap; stack:
-// (|P| + 1) level records.
+// |P| <= max out-degree.  Data region: candidate list + bitmap + label
+// hashes; stack: (|P| + 1) level records.
 uint64_t worst_data(uint64_t maxdeg, uint64_t maxout) {
   uint64_t p = std::min(maxdeg, maxout), c = maxdeg;
   uint64_t W = (p + 63) / 64;
-  return align_up(4 * c, 256) + align_up(8 * W * align_up(c, 64), 256);
+  return align_up(4 * c, 256) + align_up(8 * W * align_up(c, 64), 256) + align_up(8 * p, 256);
 }
 uint64_t worst_stack(uint64_t maxdeg, uint64_t maxout) {
   uint64_t p = std::min(maxdeg, maxout), x = maxdeg;
@@ -420,7 +452,32 @@ uint64_t worst_stack(uint64_t maxdeg, uint64_t maxout) {
   return (p + 1) * (4 * W + Wx + 4) * 8;
 }
 
-mce_status plan_create_impl(const HostGraph &g, const mce_options &opt, mce_plan **out) {
+uint32_t lds_for(const mce_plan *p, int wpc, uint32_t override_bytes) {
+  if (override_bytes) return std::min<uint32_t>(override_bytes, p->lds_per_block_max);
+  uint64_t per_wave = p->lds_per_cu / (uint64_t)std::max(1, wpc);
+  per_wave = per_wave > 1024 ? per_wave - 1024 : 0;  // static histogram + slack
+  per_wave = std::min<uint64_t>(per_wave, p->lds_per_block_max);
+  return (uint32_t)(per_wave / 256 * 256);
+}
+
+void set_config(mce_plan *p, int wpc, int share_bound, int lds_tier) {
+  p->waves_per_cu = wpc;
+  p->waves = (uint32_t)std::min<uint64_t>(p->max_waves, std::max<uint64_t>(1, (uint64_t)p->cus * wpc));
+  p->share_bound = share_bound;
+  p->lds_tier = lds_tier;
+  p->lds_bytes = lds_for(p, wpc, p->opt.lds_bytes);
+  drop_graph(p);
+}
+
+// What a plan is built from: a (possibly concatenated) graph, and for
+// batches the graph index of every vertex range.
+struct PlanSpec {
+  const HostGraph *g;
+  const std::vector<uint64_t> *graph_vstart = nullptr;  // batches: first vertex of each graph, + total
+};
+
+mce_status plan_create_impl(const PlanSpec &spec, const mce_options &opt, mce_plan **out) {
+  const HostGraph &g = *spec.g;
   auto p = new mce_plan();
   p->opt = opt;
   p->device = opt.device;
@@ -442,6 +499,8 @@ mce_status plan_create_impl(const HostGraph &g, const mce_options &opt, mce_plan
   hipDeviceProp_t prop;
   PTRY(hipGetDeviceProperties(&prop, opt.device));
   p->cus = prop.multiProcessorCount;
+  if (prop.sharedMemPerMultiprocessor) p->lds_per_cu = (uint32_t)prop.sharedMemPerMultiprocessor;
+  if (prop.sharedMemPerBlock) p->lds_per_block_max = (uint32_t)prop.sharedMemPerBlock - 1024;
   PTRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
   for (auto &e : p->ev) PTRY(hipEventCreate(&e));
 
@@ -486,24 +545,44 @@ mce_status plan_create_impl(const HostGraph &g, const mce_options &opt, mce_plan
     PTRY(dalloc(&p->d_perm, opt.n_roots));
     if (opt.n_roots) PTRY(hipMemcpy(p->d_perm, opt.roots, opt.n_roots * 8, hipMemcpyHostToDevice));
   }
+  if (opt.eager) {
+    PTRY(dalloc(&p->d_eager, p->n_dispatch));
+    if (p->n_dispatch) PTRY(hipMemcpy(p->d_eager, opt.eager, p->n_dispatch, hipMemcpyHostToDevice));
+  }
+  if (spec.graph_vstart) {
+    const auto &vs = *spec.graph_vstart;
+    p->n_graphs = vs.size() - 1;
+    p->g_hist_stride = std::max<uint32_t>(p->hist_len, kLdsHist);
+    std::vector<uint32_t> rg(p->n_roots);
+    for (uint64_t r = 0; r < p->n_roots; ++r) {
+      uint64_t v0 = mode == MCE_ROOT_EDGE ? (roots[r] >> 32) : roots[r];
+      rg[r] = (uint32_t)(std::upper_bound(vs.begin(), vs.end(), v0) - vs.begin() - 1);
+    }
+    PTRY(dalloc(&p->d_root_graph, p->n_roots));
+    if (p->n_roots) PTRY(hipMemcpy(p->d_root_graph, rg.data(), p->n_roots * 4, hipMemcpyHostToDevice));
+    PTRY(dalloc(&p->d_gcount, p->n_graphs));
+    PTRY(dalloc(&p->d_ghash, p->n_graphs));
+    PTRY(dalloc(&p->d_ghist, p->n_graphs * p->g_hist_stride));
+  }
 
   const int wpc = opt.waves_per_cu > 0 ? opt.waves_per_cu : 16;
-  p->waves = (uint32_t)std::min<uint64_t>(16384, std::max<uint64_t>(1, (uint64_t)p->cus * wpc));
+  p->max_waves = (uint32_t)std::min<uint64_t>(16384, (uint64_t)p->cus * std::max(wpc, 24));
   size_t free_b = 0, total_b = 0;
   PTRY(hipMemGetInfo(&free_b, &total_b));
   // Each wave gets a data region and a stack; both are sized for the largest
   // root when that fits in ~45% of free HBM, else capped (larger roots go to
   // the deferred pass).  slot_bytes, when given, is split evenly.
-  uint64_t cap = std::min<uint64_t>(32ull << 20, (uint64_t)(free_b * 0.22) / p->waves);
+  uint64_t cap = std::min<uint64_t>(32ull << 20, (uint64_t)(free_b * 0.22) / p->max_waves);
   uint64_t want_d = opt.slot_bytes ? opt.slot_bytes / 2 : std::min(worst_data(g.max_degree, max_out), cap);
   uint64_t want_s = opt.slot_bytes ? opt.slot_bytes / 2 : std::min(worst_stack(g.max_degree, max_out), cap);
   p->slot_bytes = align_up(std::max<uint64_t>(want_d, 2048), 256);
   p->stack_bytes = align_up(std::max<uint64_t>(want_s, 2048), 256);
-  PTRY(dalloc(&p->d_slots, p->slot_bytes * p->waves));
-  PTRY(dalloc(&p->d_stacks, p->stack_bytes * p->waves));
-  PTRY(dalloc(&p->d_recs, p->waves));
-  PTRY(dalloc(&p->d_users, p->waves));
-  PTRY(dalloc(&p->d_has_work, (p->waves + 63) / 64));
+  PTRY(dalloc(&p->d_slots, p->slot_bytes * p->max_waves));
+  PTRY(dalloc(&p->d_stacks, p->stack_bytes * p->max_waves));
+  PTRY(dalloc(&p->d_recs, p->max_waves));
+  PTRY(dalloc(&p->d_users, p->max_waves + 2));  // zeroed in whole 8-byte words
+  PTRY(dalloc(&p->d_has_work, (p->max_waves + 63) / 64));
+  set_config(p, wpc, opt.share_bound == 0 ? 24 : opt.share_bound, opt.lds_tier ? 1 : 0);
 
   PTRY(dalloc(&p->d_ctr, kNumCtr));
   PTRY(dalloc(&p->d_hist, p->hist_len));
@@ -523,34 +602,7 @@ mce_status plan_create_impl(const HostGraph &g, const mce_options &opt, mce_plan
 #undef PTRY
 }
 
-mce_status plan_run_impl(mce_plan *p, mce_result **out) {
-  auto t0 = std::chrono::steady_clock::now();
-  HIP_TRY(hipSetDevice(p->device));
-  auto res = new mce_result();
-  std::memset(res, 0, sizeof(*res));
-  auto bail = [&](mce_status s) {
-    mce_result_free(res);
-    return s;
-  };
-#define RTRY(expr)                                                                          \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess) return bail(fail(MCE_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); \
-  } while (0)
-  hipStream_t s = p->stream;
-  auto reset_sharing = [&]() -> hipError_t {
-    hipError_t e = hipMemsetAsync(p->d_recs, 0, sizeof(ShareRec) * p->waves, s);
-    if (e == hipSuccess) e = hipMemsetAsync(p->d_users, 0, 4ull * p->waves, s);
-    if (e == hipSuccess) e = hipMemsetAsync(p->d_has_work, 0, 8ull * ((p->waves + 63) / 64), s);
-    return e;
-  };
-  RTRY(hipMemsetAsync(p->d_ctr, 0, kNumCtr * 8, s));
-  RTRY(reset_sharing());
-  RTRY(hipMemsetAsync(p->d_hist, 0, p->hist_len * 8ull, s));
-  if (p->d_root_nodes) {
-    RTRY(hipMemsetAsync(p->d_root_nodes, 0, p->n_roots * 8, s));
-    RTRY(hipMemsetAsync(p->d_root_cycles, 0, p->n_roots * 8, s));
-  }
+KArgs make_args(const mce_plan *p) {
   KArgs a{};
   a.rowoff = p->d_rowoff;
   a.col = p->d_col;
@@ -560,6 +612,7 @@ mce_status plan_run_impl(mce_plan *p, mce_result **out) {
   a.roots = p->d_roots;
   a.n_roots_canon = p->n_roots;
   a.perm = p->d_perm;
+  a.eager = p->d_eager;
   a.n_dispatch = p->n_dispatch;
   a.reverse = p->d_perm ? 0 : 1;
   a.root_mode = p->opt.root_mode;
@@ -577,13 +630,6 @@ mce_status plan_run_impl(mce_plan *p, mce_result **out) {
   a.active = (unsigned int *)(p->d_ctr + kActive);
   a.idle = (unsigned int *)(p->d_ctr + kIdle);
   a.err = (unsigned int *)(p->d_ctr + kErr);
-  a.stacks = p->d_stacks;
-  a.stack_bytes = p->stack_bytes;
-  a.recs = p->d_recs;
-  a.users = p->d_users;
-  a.has_work = p->d_has_work;
-  a.share_bound = p->opt.share_bound == 0 ? 24 : p->opt.share_bound;
-  a.n_waves = p->waves;
   a.hist = p->d_hist;
   a.hist_len = p->hist_len;
   a.root_nodes = p->d_root_nodes;
@@ -592,91 +638,190 @@ mce_status plan_run_impl(mce_plan *p, mce_result **out) {
   a.list_cap = p->list_cap;
   a.slots = p->d_slots;
   a.slot_bytes = p->slot_bytes;
+  a.stacks = p->d_stacks;
+  a.stack_bytes = p->stack_bytes;
+  a.recs = p->d_recs;
+  a.users = p->d_users;
+  a.has_work = p->d_has_work;
+  a.share_bound = p->share_bound;
+  a.n_waves = p->waves;
+  a.lds_tier = p->lds_tier;
+  a.lds_bytes = p->lds_bytes;
   a.deferred = p->d_deferred;
   a.deferred_cap = p->deferred_cap;
+  a.root_graph = p->d_root_graph;
+  a.g_count = p->d_gcount;
+  a.g_hash = p->d_ghash;
+  a.g_hist = p->d_ghist;
+  a.g_hist_stride = p->g_hist_stride;
+  return a;
+}
 
+// Resets every counter, result and sharing word of a run with fill kernels
+// (graph-capturable) and launches the enumeration kernel.
+hipError_t enqueue_run(const mce_plan *p, const KArgs &a, hipStream_t s) {
+  hipError_t e = launch_fill_zero(p->d_ctr, kNumCtr, s);
+  if (e == hipSuccess) e = launch_fill_zero(p->d_hist, p->hist_len, s);
+  if (e == hipSuccess) e = launch_fill_zero(p->d_recs, sizeof(ShareRec) / 8 * p->waves, s);
+  if (e == hipSuccess) e = launch_fill_zero(p->d_users, (p->waves + 1) / 2, s);
+  if (e == hipSuccess) e = launch_fill_zero(p->d_has_work, (p->waves + 63) / 64, s);
+  if (e == hipSuccess && p->d_root_nodes) e = launch_fill_zero(p->d_root_nodes, p->n_roots, s);
+  if (e == hipSuccess && p->d_root_cycles) e = launch_fill_zero(p->d_root_cycles, p->n_roots, s);
+  if (e == hipSuccess && p->d_gcount) e = launch_fill_zero(p->d_gcount, p->n_graphs, s);
+  if (e == hipSuccess && p->d_ghash) e = launch_fill_zero(p->d_ghash, p->n_graphs, s);
+  if (e == hipSuccess && p->d_ghist) e = launch_fill_zero(p->d_ghist, p->n_graphs * p->g_hist_stride, s);
+  if (e == hipSuccess) e = launch_mce(a, p->waves, s);
+  return e;
+}
+
+struct RunOut {
+  unsigned long long ctr[kNumCtr];
   float kernel_ms = 0;
-  res->passes = 0;
+  int passes = 0;
+  bool replay = false;
+};
+
+// One enumeration (plus the deferred pass when needed).  `a` may carry a
+// dispatch list other than the plan's (autotuner samples).
+mce_status run_core(mce_plan *p, const KArgs &a0, bool allow_graph, RunOut &ro) {
+  hipStream_t s = p->stream;
+  KArgs a = a0;
+  ro.passes = 0;
+  ro.kernel_ms = 0;
+  if (a.n_dispatch == 0) {
+    HIP_TRY(launch_fill_zero(p->d_ctr, kNumCtr, s));
+    HIP_TRY(launch_fill_zero(p->d_hist, p->hist_len, s));
+    if (p->d_gcount) {
+      HIP_TRY(launch_fill_zero(p->d_gcount, p->n_graphs, s));
+      HIP_TRY(launch_fill_zero(p->d_ghash, p->n_graphs, s));
+      HIP_TRY(launch_fill_zero(p->d_ghist, p->n_graphs * p->g_hist_stride, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    std::memset(ro.ctr, 0, sizeof(ro.ctr));
+    return MCE_OK;
+  }
+  const bool use_graph = allow_graph && p->opt.use_graph;
+  if (use_graph && !p->graph_exec) {
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_run(p, a, s);
+    hipGraph_t gr = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s, &gr);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return fail(MCE_ERR_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(e ? e : e2));
+    }
+    p->graph = gr;
+    HIP_TRY(hipGraphInstantiate(&p->graph_exec, p->graph, nullptr, nullptr, 0));
+  }
+  HIP_TRY(hipEventRecord(p->ev[0], s));
+  if (use_graph) {
+    HIP_TRY(hipGraphLaunch(p->graph_exec, s));
+    ro.replay = true;
+  } else {
+    HIP_TRY(enqueue_run(p, a, s));
+  }
+  HIP_TRY(hipEventRecord(p->ev[1], s));
+  HIP_TRY(hipEventSynchronize(p->ev[1]));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
+  ro.kernel_ms += ms;
+  ro.passes = 1;
+  HIP_TRY(hipMemcpy(ro.ctr, p->d_ctr, sizeof(ro.ctr), hipMemcpyDeviceToHost));
+  uint64_t nd = ro.ctr[kNDeferred];
+  if (nd > p->deferred_cap) return fail(MCE_ERR_NOMEM, "deferred-root list overflow");
+  if (nd) {
+    // Second pass: the deferred roots with scratch sized to the largest need.
+    uint64_t slot2 = align_up(std::max<uint64_t>(ro.ctr[kDeferNeed], 2048), 256);
+    uint64_t stack2 = align_up(std::max<uint64_t>(ro.ctr[kDeferStackNeed], 2048), 256);
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    uint64_t fit = (uint64_t)(free_b * 0.8) / (slot2 + stack2);
+    uint64_t waves2 = std::min<uint64_t>({(uint64_t)p->waves, nd, fit});
+    if (waves2 == 0) return fail(MCE_ERR_NOMEM, "a root needs " + std::to_string(slot2 + stack2) + " bytes of scratch");
+    if (p->slots2_bytes < slot2 * waves2) {
+      if (p->d_slots2) (void)hipFree(p->d_slots2);
+      p->d_slots2 = nullptr;
+      p->slots2_bytes = 0;
+      HIP_TRY(dalloc(&p->d_slots2, slot2 * waves2));
+      p->slots2_bytes = slot2 * waves2;
+    }
+    if (p->stacks2_bytes < stack2 * waves2) {
+      if (p->d_stacks2) (void)hipFree(p->d_stacks2);
+      p->d_stacks2 = nullptr;
+      p->stacks2_bytes = 0;
+      HIP_TRY(dalloc(&p->d_stacks2, stack2 * waves2));
+      p->stacks2_bytes = stack2 * waves2;
+    }
+    if (!p->d_deferred2) HIP_TRY(dalloc(&p->d_deferred2, p->deferred_cap));
+    HIP_TRY(hipMemcpyAsync(p->d_deferred2, p->d_deferred, nd * 8, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(launch_fill_zero(p->d_ctr + kNext, 1, s));
+    HIP_TRY(launch_fill_zero(p->d_ctr + kNDeferred, 3, s));
+    HIP_TRY(launch_fill_zero(p->d_ctr + kActive, 2, s));
+    HIP_TRY(launch_fill_zero(p->d_recs, sizeof(ShareRec) / 8 * waves2, s));
+    HIP_TRY(launch_fill_zero(p->d_users, (waves2 + 1) / 2, s));
+    HIP_TRY(launch_fill_zero(p->d_has_work, (waves2 + 63) / 64, s));
+    KArgs b = a;
+    b.perm = p->d_deferred2;
+    b.eager = nullptr;
+    b.n_dispatch = nd;
+    b.reverse = 0;
+    b.slots = p->d_slots2;
+    b.slot_bytes = slot2;
+    b.stacks = p->d_stacks2;
+    b.stack_bytes = stack2;
+    b.n_waves = (uint32_t)waves2;
+    HIP_TRY(hipEventRecord(p->ev[2], s));
+    HIP_TRY(launch_mce(b, (uint32_t)waves2, s));
+    HIP_TRY(hipEventRecord(p->ev[3], s));
+    HIP_TRY(hipEventSynchronize(p->ev[3]));
+    HIP_TRY(hipEventElapsedTime(&ms, p->ev[2], p->ev[3]));
+    ro.kernel_ms += ms;
+    ro.passes = 2;
+    unsigned long long c2[kNumCtr];
+    HIP_TRY(hipMemcpy(c2, p->d_ctr, sizeof(c2), hipMemcpyDeviceToHost));
+    if (c2[kNDeferred]) return fail(MCE_ERR_INTERNAL, "root deferred twice");
+    uint64_t deferred_first = ro.ctr[kNDeferred];
+    std::memcpy(ro.ctr, c2, sizeof(c2));
+    ro.ctr[kNDeferred] = deferred_first;
+  }
+  uint32_t err = (uint32_t)ro.ctr[kErr];
+  if (err & kErrPTooLarge) return fail(MCE_ERR_UNSUPPORTED, "a root has more than 4096 candidates");
+  if (err & kErrHistOverflow) return fail(MCE_ERR_INTERNAL, "clique size beyond histogram");
+  if (err & kErrListFull) return fail(MCE_ERR_NOMEM, "clique list capacity exceeded");
+  if (err & kErrDeferFull) return fail(MCE_ERR_NOMEM, "deferred-root list overflow");
+  return MCE_OK;
+}
+
+mce_status plan_run_impl(mce_plan *p, mce_result **out) {
+  auto t0 = std::chrono::steady_clock::now();
+  HIP_TRY(hipSetDevice(p->device));
+  RunOut ro;
+  mce_status st = run_core(p, make_args(p), true, ro);
+  if (st) return st;
+  auto res = new mce_result();
+  std::memset(res, 0, sizeof(*res));
+  auto bail = [&](mce_status s) {
+    mce_result_free(res);
+    return s;
+  };
+#define RTRY(expr)                                                                          \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return bail(fail(MCE_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+  } while (0)
+  res->passes = ro.passes;
   res->waves = (int32_t)p->waves;
-  if (p->n_dispatch) {
-    RTRY(hipEventRecord(p->ev[0], s));
-    RTRY(launch_mce(a, p->waves, s));
-    RTRY(hipEventRecord(p->ev[1], s));
-    RTRY(hipEventSynchronize(p->ev[1]));
-    float ms = 0;
-    RTRY(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
-    kernel_ms += ms;
-    res->passes = 1;
-    unsigned long long ctr[kNumCtr];
-    RTRY(hipMemcpy(ctr, p->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
-    uint64_t nd = ctr[kNDeferred];
-    if (nd > p->deferred_cap) return bail(fail(MCE_ERR_NOMEM, "deferred-root list overflow"));
-    if (nd) {
-      // Second pass: the deferred roots with slots sized to the largest need.
-      res->deferred_roots = nd;
-      uint64_t slot2 = align_up(std::max<uint64_t>(ctr[kDeferNeed], 2048), 256);
-      uint64_t stack2 = align_up(std::max<uint64_t>(ctr[kDeferStackNeed], 2048), 256);
-      size_t free_b = 0, total_b = 0;
-      RTRY(hipMemGetInfo(&free_b, &total_b));
-      uint64_t fit = (uint64_t)(free_b * 0.8) / (slot2 + stack2);
-      uint64_t waves2 = std::min<uint64_t>({(uint64_t)p->waves, nd, fit});
-      if (waves2 == 0)
-        return bail(fail(MCE_ERR_NOMEM, "a root needs " + std::to_string(slot2) + " bytes of scratch"));
-      if (p->slots2_bytes < slot2 * waves2) {
-        if (p->d_slots2) hipFree(p->d_slots2);
-        p->d_slots2 = nullptr;
-        p->slots2_bytes = 0;
-        RTRY(dalloc(&p->d_slots2, slot2 * waves2));
-        p->slots2_bytes = slot2 * waves2;
-      }
-      if (p->stacks2_bytes < stack2 * waves2) {
-        if (p->d_stacks2) hipFree(p->d_stacks2);
-        p->d_stacks2 = nullptr;
-        p->stacks2_bytes = 0;
-        RTRY(dalloc(&p->d_stacks2, stack2 * waves2));
-        p->stacks2_bytes = stack2 * waves2;
-      }
-      RTRY(reset_sharing());
-      if (!p->d_deferred2) RTRY(dalloc(&p->d_deferred2, p->deferred_cap));
-      RTRY(hipMemcpyAsync(p->d_deferred2, p->d_deferred, nd * 8, hipMemcpyDeviceToDevice, s));
-      RTRY(hipMemsetAsync(p->d_ctr + kNext, 0, 8, s));
-      RTRY(hipMemsetAsync(p->d_ctr + kNDeferred, 0, 8 * 3, s));
-      RTRY(hipMemsetAsync(p->d_ctr + kActive, 0, 8 * 2, s));
-      KArgs b = a;
-      b.perm = p->d_deferred2;
-      b.n_dispatch = nd;
-      b.reverse = 0;
-      b.slots = p->d_slots2;
-      b.slot_bytes = slot2;
-      b.stacks = p->d_stacks2;
-      b.stack_bytes = stack2;
-      b.n_waves = (uint32_t)waves2;
-      RTRY(hipEventRecord(p->ev[2], s));
-      RTRY(launch_mce(b, (uint32_t)waves2, s));
-      RTRY(hipEventRecord(p->ev[3], s));
-      RTRY(hipEventSynchronize(p->ev[3]));
-      RTRY(hipEventElapsedTime(&ms, p->ev[2], p->ev[3]));
-      kernel_ms += ms;
-      res->passes = 2;
-      RTRY(hipMemcpy(ctr, p->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
-      if (ctr[kNDeferred]) return bail(fail(MCE_ERR_INTERNAL, "root deferred twice"));
-    }
-    uint32_t err = (uint32_t)ctr[kErr];
-    if (err & kErrPTooLarge) return bail(fail(MCE_ERR_UNSUPPORTED, "a root has more than 4096 candidates"));
-    if (err & kErrHistOverflow) return bail(fail(MCE_ERR_INTERNAL, "clique size beyond histogram"));
-    if (err & kErrListFull) return bail(fail(MCE_ERR_NOMEM, "clique list capacity exceeded"));
-    if (err & kErrDeferFull) return bail(fail(MCE_ERR_NOMEM, "deferred-root list overflow"));
-    res->count = ctr[kCount];
-    res->hash = ctr[kHash];
-    res->total_nodes = ctr[kNodes];
-    res->shared_tasks = ctr[kShared];
-    res->search_bytes = ctr[kWords] * 8;
-    res->build_bytes = ctr[kBuild];
-    if (p->d_list) {
-      res->list_len = ctr[kListCursor];
-      res->list = (uint32_t *)std::malloc(std::max<uint64_t>(res->list_len, 1) * 4);
-      if (res->list_len) RTRY(hipMemcpy(res->list, p->d_list, res->list_len * 4, hipMemcpyDeviceToHost));
-    }
+  res->deferred_roots = ro.ctr[kNDeferred];
+  res->count = ro.ctr[kCount];
+  res->hash = ro.ctr[kHash];
+  res->total_nodes = ro.ctr[kNodes];
+  res->shared_tasks = ro.ctr[kShared];
+  res->search_bytes = ro.ctr[kWords] * 8;
+  res->build_bytes = ro.ctr[kBuild];
+  if (p->d_list) {
+    res->list_len = ro.ctr[kListCursor];
+    res->list = (uint32_t *)std::malloc(std::max<uint64_t>(res->list_len, 1) * 4);
+    if (res->list_len) RTRY(hipMemcpy(res->list, p->d_list, res->list_len * 4, hipMemcpyDeviceToHost));
   }
   res->hist_len = p->hist_len;
   res->hist = (uint64_t *)std::calloc(p->hist_len, 8);
@@ -690,11 +835,93 @@ mce_status plan_run_impl(mce_plan *p, mce_result **out) {
       RTRY(hipMemcpy(res->root_cycles, p->d_root_cycles, p->n_roots * 8, hipMemcpyDeviceToHost));
     }
   }
-  res->kernel_ms = kernel_ms;
+  res->kernel_ms = ro.kernel_ms;
   res->total_ms = ms_since(t0);
   *out = res;
   return MCE_OK;
 #undef RTRY
+}
+
+mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
+  HIP_TRY(hipSetDevice(p->device));
+  if (stride == 0) stride = 1;
+  // Sample: every stride-th dispatched root.
+  std::vector<uint64_t> disp(p->n_dispatch);
+  if (p->d_perm) HIP_TRY(hipMemcpy(disp.data(), p->d_perm, p->n_dispatch * 8, hipMemcpyDeviceToHost));
+  else
+    for (uint64_t i = 0; i < p->n_dispatch; ++i) disp[i] = p->n_dispatch - 1 - i;
+  std::vector<uint64_t> sample;
+  for (uint64_t i = 0; i < disp.size(); i += stride) sample.push_back(disp[i]);
+  uint64_t *d_sample = nullptr;
+  HIP_TRY(dalloc(&d_sample, sample.size()));
+  if (!sample.empty()) HIP_TRY(hipMemcpy(d_sample, sample.data(), sample.size() * 8, hipMemcpyHostToDevice));
+  const int wpcs[] = {8, 12, 16, 20, 24};
+  const int taus[] = {8, 16, 24, 48, -1};
+  const int base_wpc = p->waves_per_cu, base_tau = p->share_bound, base_lds = p->lds_tier;
+  double best = 1e30, base_ms = 0;
+  int best_wpc = base_wpc, best_tau = base_tau, best_lds = base_lds, trials = 0;
+  unsigned long long ref_count = 0, ref_hash = 0;
+  auto trial = [&](int wpc, int tau, int lds, double &ms) -> mce_status {
+    set_config(p, wpc, tau, lds);
+    KArgs a = make_args(p);
+    a.perm = d_sample;
+    a.eager = nullptr;
+    a.n_dispatch = sample.size();
+    a.reverse = 0;
+    RunOut ro;
+    mce_status s = run_core(p, a, false, ro);
+    if (s) return s;
+    ms = ro.kernel_ms;
+    if (trials == 0) {
+      ref_count = ro.ctr[kCount];
+      ref_hash = ro.ctr[kHash];
+    } else if (ro.ctr[kCount] != ref_count || ro.ctr[kHash] != ref_hash) {
+      return fail(MCE_ERR_INTERNAL, "autotuner settings disagree on the result");
+    }
+    ++trials;
+    return MCE_OK;
+  };
+  mce_status st = MCE_OK;
+  double ms = 0;
+  // default setting first (warm-up run, then the timed one)
+  st = trial(base_wpc, base_tau, base_lds, ms);
+  if (!st) st = trial(base_wpc, base_tau, base_lds, base_ms);
+  if (!st) best = base_ms;
+  for (int lds = 1; lds >= 0 && !st; --lds)
+    for (int wpc : wpcs) {
+      if ((uint64_t)p->cus * wpc > p->max_waves || st) continue;
+      if (!st) st = trial(wpc, base_tau, lds, ms);
+      if (!st && ms < best) {
+        best = ms;
+        best_wpc = wpc;
+        best_lds = lds;
+        best_tau = base_tau;
+      }
+    }
+  for (int tau : taus) {
+    if (st) break;
+    st = trial(best_wpc, tau, best_lds, ms);
+    if (!st && ms < best) {
+      best = ms;
+      best_tau = tau;
+    }
+  }
+  (void)hipFree(d_sample);
+  if (st) {
+    set_config(p, base_wpc, base_tau, base_lds);
+    return st;
+  }
+  set_config(p, best_wpc, best_tau, best_lds);
+  if (out) {
+    std::memset(out, 0, sizeof(*out));
+    out->waves_per_cu = best_wpc;
+    out->share_bound = best_tau;
+    out->lds_tier = best_lds;
+    out->trials = trials;
+    out->best_ms = best;
+    out->base_ms = base_ms;
+  }
+  return MCE_OK;
 }
 }  // namespace
 
@@ -706,7 +933,9 @@ mce_status mce_plan_create(const mce_graph *g, const mce_options *opt, mce_plan 
     mce_options o;
     mce_options_init(&o);
     if (opt) o = *opt;
-    return plan_create_impl(g->g, o, out);
+    PlanSpec spec;
+    spec.g = &g->g;
+    return plan_create_impl(spec, o, out);
   });
 }
 
@@ -714,6 +943,13 @@ mce_status mce_plan_run(mce_plan *p, mce_result **out) {
   return guarded([&] {
     if (!p || !out) return fail(MCE_ERR_ARG, "null argument");
     return plan_run_impl(p, out);
+  });
+}
+
+mce_status mce_plan_autotune(mce_plan *p, uint64_t stride, mce_tune_result *out) {
+  return guarded([&] {
+    if (!p) return fail(MCE_ERR_ARG, "null plan");
+    return autotune_impl(p, stride, out);
   });
 }
 
@@ -765,6 +1001,117 @@ void mce_result_free(mce_result *r) {
   std::free(r->root_nodes);
   std::free(r->root_cycles);
   delete r;
+}
+
+// ---------------------------------------------------------------------------
+// Batches
+// ---------------------------------------------------------------------------
+mce_status mce_batch_create(const mce_graph *const *graphs, uint64_t count, const mce_options *opt,
+                            mce_batch **out) {
+  return guarded([&] {
+    if (!graphs || !out || count == 0) return fail(MCE_ERR_ARG, "empty batch");
+    mce_options o;
+    mce_options_init(&o);
+    if (opt) o = *opt;
+    if (o.schedule == MCE_SCHED_CUSTOM && !o.roots) return fail(MCE_ERR_ARG, "custom schedule without a root list");
+    if (o.want_list) return fail(MCE_ERR_UNSUPPORTED, "clique lists are per-graph solves only");
+    // Concatenate: graph i's vertices become [vstart[i], vstart[i+1]).
+    HostGraph cat;
+    std::vector<uint64_t> vstart(count + 1, 0);
+    uint64_t nnz = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+      if (!graphs[i]) return fail(MCE_ERR_ARG, "null graph in batch");
+      vstart[i + 1] = vstart[i] + graphs[i]->g.n;
+      nnz += graphs[i]->g.col.size();
+    }
+    if (vstart[count] >= (1ull << 32)) return fail(MCE_ERR_UNSUPPORTED, "batch has more than 2^32-1 vertices");
+    cat.n = vstart[count];
+    cat.m = nnz / 2;
+    cat.rowoff.resize(cat.n + 1);
+    cat.col.resize(nnz);
+    cat.labels.resize(cat.n);
+    uint64_t at = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+      const HostGraph &g = graphs[i]->g;
+      const uint32_t base = (uint32_t)vstart[i];
+      for (uint64_t v = 0; v < g.n; ++v) {
+        cat.rowoff[base + v] = at + g.rowoff[v];
+        cat.labels[base + v] = g.labels[v];
+      }
+      for (uint64_t k = 0; k < g.col.size(); ++k) cat.col[at + k] = g.col[k] + base;
+      at += g.col.size();
+      cat.max_degree = std::max(cat.max_degree, g.max_degree);
+      cat.degeneracy = std::max(cat.degeneracy, g.degeneracy);
+    }
+    cat.rowoff[cat.n] = at;
+    auto b = new mce_batch();
+    PlanSpec spec;
+    spec.g = &cat;
+    spec.graph_vstart = &vstart;
+    mce_status s = plan_create_impl(spec, o, &b->plan);
+    if (s) {
+      delete b;
+      return s;
+    }
+    *out = b;
+    return MCE_OK;
+  });
+}
+
+mce_status mce_batch_run(mce_batch *b, mce_batch_result **out) {
+  return guarded([&] {
+    if (!b || !out) return fail(MCE_ERR_ARG, "null argument");
+    auto t0 = std::chrono::steady_clock::now();
+    mce_plan *p = b->plan;
+    HIP_TRY(hipSetDevice(p->device));
+    RunOut ro;
+    mce_status st = run_core(p, make_args(p), true, ro);
+    if (st) return st;
+    auto r = new mce_batch_result();
+    std::memset(r, 0, sizeof(*r));
+    r->n_graphs = p->n_graphs;
+    r->hist_stride = p->g_hist_stride;
+    r->count = (uint64_t *)std::calloc(std::max<uint64_t>(p->n_graphs, 1), 8);
+    r->hash = (uint64_t *)std::calloc(std::max<uint64_t>(p->n_graphs, 1), 8);
+    r->hist = (uint64_t *)std::calloc(std::max<uint64_t>(p->n_graphs * p->g_hist_stride, 1), 8);
+    hipError_t e = hipMemcpy(r->count, p->d_gcount, p->n_graphs * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(r->hash, p->d_ghash, p->n_graphs * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy(r->hist, p->d_ghist, p->n_graphs * p->g_hist_stride * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      mce_batch_result_free(r);
+      return fail(MCE_ERR_DEVICE, hipGetErrorString(e));
+    }
+    for (uint64_t i = 0; i < p->n_graphs; ++i) r->total_count += r->count[i];
+    r->total_nodes = ro.ctr[kNodes];
+    r->kernel_ms = ro.kernel_ms;
+    r->total_ms = ms_since(t0);
+    r->graph_replay = ro.replay ? 1 : 0;
+    r->passes = ro.passes;
+    *out = r;
+    return MCE_OK;
+  });
+}
+
+mce_status mce_batch_autotune(mce_batch *b, uint64_t stride, mce_tune_result *out) {
+  return guarded([&] {
+    if (!b) return fail(MCE_ERR_ARG, "null batch");
+    return autotune_impl(b->plan, stride, out);
+  });
+}
+
+void mce_batch_result_free(mce_batch_result *r) {
+  if (!r) return;
+  std::free(r->count);
+  std::free(r->hash);
+  std::free(r->hist);
+  delete r;
+}
+
+void mce_batch_free(mce_batch *b) {
+  if (!b) return;
+  plan_release(b->plan);
+  delete b;
 }
 
 }  // extern "C"

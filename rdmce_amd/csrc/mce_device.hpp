@@ -17,13 +17,16 @@ constexpr uint32_t kErrHistOverflow = 2u;    // clique larger than hist_len - 1
 constexpr uint32_t kErrDeferFull = 4u;       // deferred-root list full
 constexpr uint32_t kErrListFull = 8u;        // clique list capacity exceeded
 
-// One root's subproblem, as every wavefront working on it sees it.
+// One root's subproblem, as every wavefront working on it sees it.  The
+// pointers are HBM copies (the owner may work on an LDS copy of bm and gP).
 struct Sub {
   uint32_t p, xq, W, Wx, rows_pad, LS;
   uint32_t base_size, v0, v1, bm_owner;
+  uint32_t levels, graph;  // graph: index within a batch (0 outside batches)
   uint64_t base_key, canon;
   const uint32_t *Plist;  // p entries, P0 ascending
-  const uint64_t *bm;     // word-major bitmap
+  const uint64_t *bm;     // word-major bitmap, W * rows_pad words
+  const uint64_t *gP;     // splitmix64(label) of each P0 vertex, p words
 };
 
 // Exposure record of one wavefront: at most one published level at a time.
@@ -50,6 +53,7 @@ struct KArgs {
   // Dispatch: entry i of this pass handles canonical root
   //   perm ? perm[i] : (reverse ? n_dispatch-1-i : i).
   const uint64_t *perm;
+  const uint8_t *eager;         // per dispatch index: expose the top level at once
   uint64_t n_dispatch;
   int32_t reverse;
   int32_t root_mode;            // 1 vertex, 2 edge
@@ -82,6 +86,10 @@ struct KArgs {
   unsigned long long *shared_tasks; // tasks taken by a wave other than the publisher
   int32_t share_bound;              // |P| threshold for exposing a level; <= 0: off
   uint32_t n_waves;
+  // LDS tier: subproblems whose bitmap, label hashes and stack fit in
+  // lds_bytes are searched entirely in the wave's LDS.
+  int32_t lds_tier;
+  uint32_t lds_bytes;
   // Roots whose scratch need exceeds the slots are deferred to a pass with
   // larger slots: canonical ids, count, and the largest needs seen.
   uint64_t *deferred;
@@ -90,9 +98,19 @@ struct KArgs {
   unsigned long long *deferred_need;        // data region bytes
   unsigned long long *deferred_stack_need;  // stack bytes
   unsigned int *err;
+  // Batches: graph index of each canonical root (nullable) and per-graph
+  // results (count, hash, hist with `g_hist_stride` entries per graph).
+  const uint32_t *root_graph;
+  unsigned long long *g_count;
+  unsigned long long *g_hash;
+  unsigned long long *g_hist;
+  uint32_t g_hist_stride;
 };
 
 hipError_t launch_mce(const KArgs &a, uint32_t waves, hipStream_t stream);
 hipError_t launch_copy_probe(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
+// Zero-fills n 64-bit words with a kernel (safe inside captured hipGraphs,
+// unlike memset nodes).
+hipError_t launch_fill_zero(void *p, uint64_t n_words, hipStream_t stream);
 
 }  // namespace rdmce

@@ -25,6 +25,8 @@
 //    not fit the wave's slot is deferred to a second pass with larger slots
 //    instead of being dropped (the reference drops it,
 //    BkPivotBitBalance.cuh:480-487).
+#include <type_traits>
+
 #include "mce_device.hpp"
 #include "rng.hpp"
 
@@ -145,7 +147,10 @@ __device__ __forceinline__ void emit(const KArgs &a, const Sub &s, Acc &acc, uns
     acc.count++;
     acc.hash += splitmix64_mix(key);
     if (size < kLdsHist) s_hist[size]++;
-    else if (size < a.hist_len) atomicAdd(&a.hist[size], 1ull);
+    else if (a.root_graph) {
+      if (size < a.g_hist_stride) atomicAdd(&a.g_hist[(uint64_t)s.graph * a.g_hist_stride + size], 1ull);
+      else atomicOr(a.err, kErrHistOverflow);
+    } else if (size < a.hist_len) atomicAdd(&a.hist[size], 1ull);
     else {
       atomicAdd(&a.hist[a.hist_len - 1], 1ull);
       atomicOr(a.err, kErrHistOverflow);
@@ -173,6 +178,9 @@ __device__ __forceinline__ void emit(const KArgs &a, const Sub &s, Acc &acc, uns
   }
 }
 
+// LDS pointers (address space 3) for the LDS-tier instantiations.
+using lds64 = __attribute__((address_space(3))) uint64_t;
+
 // Level record layout (u64 words): R[W] P[W] XP[W] V[W] XX[Wx] meta[4]
 //   meta[0] = key (sum of label hashes of the clique so far)
 //   meta[1] = rsize | xxcnt << 32
@@ -184,7 +192,8 @@ struct Lvl {
   uint32_t rsize, xxcnt, xlo, xhi;
 };
 
-__device__ __forceinline__ void store_rpx(uint64_t *rec, const Sub &s, const Lvl &L) {
+template <class Ptr>
+__device__ __forceinline__ void store_rpx(Ptr rec, const Sub &s, const Lvl &L) {
   const uint32_t l = lane(), W = s.W;
   if (l < W) {
     rec[l] = L.rk;
@@ -192,14 +201,14 @@ __device__ __forceinline__ void store_rpx(uint64_t *rec, const Sub &s, const Lvl
     rec[2 * W + l] = L.xpk;
   }
   if (l == 0) {
-    uint64_t *meta = rec + 4 * W + s.Wx;
-    meta[0] = L.key;
-    meta[1] = (uint64_t)L.rsize | ((uint64_t)L.xxcnt << 32);
-    meta[2] = (uint64_t)L.xlo | ((uint64_t)L.xhi << 32);
+    rec[4 * W + s.Wx] = L.key;
+    rec[4 * W + s.Wx + 1] = (uint64_t)L.rsize | ((uint64_t)L.xxcnt << 32);
+    rec[4 * W + s.Wx + 2] = (uint64_t)L.xlo | ((uint64_t)L.xhi << 32);
   }
 }
 
-__device__ __forceinline__ Lvl load_level(const uint64_t *rec, const Sub &s) {
+template <class Ptr>
+__device__ __forceinline__ Lvl load_level(Ptr rec, const Sub &s) {
   const uint32_t l = lane(), W = s.W;
   Lvl L;
   if (l < W) {
@@ -210,9 +219,8 @@ __device__ __forceinline__ Lvl load_level(const uint64_t *rec, const Sub &s) {
   } else {
     L.rk = L.pk = L.xpk = L.vk = 0;
   }
-  const uint64_t *meta = rec + 4 * W + s.Wx;
-  L.key = meta[0];
-  uint64_t m1 = meta[1], m2 = meta[2];
+  L.key = rec[4 * W + s.Wx];
+  uint64_t m1 = rec[4 * W + s.Wx + 1], m2 = rec[4 * W + s.Wx + 2];
   L.rsize = (uint32_t)m1;
   L.xxcnt = (uint32_t)(m1 >> 32);
   L.xlo = (uint32_t)m2;
@@ -226,11 +234,11 @@ __device__ __forceinline__ Lvl load_level(const uint64_t *rec, const Sub &s) {
 // (BkPivotBitBalance.cuh:351-382); XP' and XX' are then restricted to the
 // common neighbours of the moved vertices.  Writes the child's record to
 // `out` and returns it.
-__device__ Lvl make_child(const KArgs &a, const Sub &s, const Lvl &par, const uint64_t *par_xx, uint32_t c,
-                          uint64_t *out, Acc &acc) {
+template <class BmP, class GpP, class XxP, class OutP>
+__device__ Lvl make_child(const Sub &s, const Lvl &par, XxP par_xx, uint32_t c, OutP out, BmP bm, GpP gP,
+                          Acc &acc) {
   const uint32_t l = lane(), W = s.W, rp = s.rows_pad, p = s.p;
   const bool wl = l < W;
-  const uint64_t *bm = s.bm;
   const uint32_t kc = c >> 6;
   const uint64_t cbit = 1ull << (c & 63);
   uint64_t rowc = wl ? bm[(uint64_t)l * rp + c] : 0;
@@ -240,7 +248,7 @@ __device__ Lvl make_child(const KArgs &a, const Sub &s, const Lvl &par, const ui
   ch.pk = par.pk & rowc;
   ch.xpk = par.xpk & rowc;
   ch.vk = 0;
-  ch.key = par.key + a.label_hash[s.Plist[c]];
+  ch.key = par.key + gP[c];
   ch.rsize = par.rsize + 1;
   uint64_t mk = 0;
   uint32_t psz = wave_sum_u32((uint32_t)__popcll(ch.pk));
@@ -269,14 +277,13 @@ __device__ Lvl make_child(const KArgs &a, const Sub &s, const Lvl &par, const ui
           uint32_t mm = j * 64 + bb;
           acc.words += W;
           if (wl) ch.xpk &= bm[(uint64_t)l * rp + mm];
-          ch.key += a.label_hash[s.Plist[mm]];
+          ch.key += gP[mm];
           ch.rsize++;
         }
       }
     }
   }
   const bool anym = __ballot(mk != 0) != 0;
-  uint64_t *xx2 = out + 4 * W;
   const uint32_t cb = c & 63;
   uint32_t cnt2 = 0, nlo = par.xhi, nhi = par.xlo;
   for (uint32_t j = par.xlo; j < par.xhi; ++j) {
@@ -294,7 +301,7 @@ __device__ Lvl make_child(const KArgs &a, const Sub &s, const Lvl &par, const ui
       }
       nw = __ballot(hit);
     }
-    if (l == 0) xx2[j] = nw;
+    if (l == 0) out[4 * W + j] = nw;
     if (nw) {
       cnt2 += __popcll(nw);
       nlo = j < nlo ? j : nlo;
@@ -326,8 +333,8 @@ __device__ __forceinline__ int claim_task(ShareRec *r, bool *last) {
 // candidate t of V, with P and XP as sequential DFS would hold them when it
 // reaches t (candidates of V below t moved from P to XP) - the same child
 // the publisher would have generated (BkPivotBitBalance.cuh:40-70).
-__device__ Lvl materialize(const KArgs &a, const Sub &s, const uint64_t *snap, uint32_t idx, uint64_t *out,
-                           Acc &acc) {
+template <class BmP, class GpP, class SnapP, class OutP>
+__device__ Lvl materialize(const Sub &s, SnapP snap, uint32_t idx, OutP out, BmP bm, GpP gP, Acc &acc) {
   const uint32_t l = lane(), W = s.W;
   const bool wl = l < W;
   Lvl par = load_level(snap, s);
@@ -347,7 +354,7 @@ __device__ Lvl materialize(const KArgs &a, const Sub &s, const uint64_t *snap, u
   uint64_t below = l < jt ? par.vk : (l == jt ? (par.vk & low_mask(bt)) : 0);
   par.pk &= ~below;
   par.xpk |= below;
-  return make_child(a, s, par, snap + 4 * W, jt * 64 + bt, out, acc);
+  return make_child(s, par, snap + 4 * W, jt * 64 + bt, out, bm, gP, acc);
 }
 
 __device__ __forceinline__ void publish_bit(const KArgs &a, uint32_t w, bool set) {
@@ -356,18 +363,22 @@ __device__ __forceinline__ void publish_bit(const KArgs &a, uint32_t w, bool set
 }
 
 // Depth-first search from the record at level 0 of `stk` (a root or a
-// materialized task).  Exposes levels root-down while other waves are idle
+// materialized task).  bm, gP and stk are either this wave's LDS copies or
+// HBM; stk_g is this wave's HBM stack arena (frozen levels are copied there
+// when the stack lives in LDS, so other waves can read them).
+// Exposes levels root-down while other waves are idle
 // (BkPivotBitBalance.cuh:87-138): when a branching node has |P| >=
 // share_bound, the shallowest not yet exposed ancestor level is frozen and
 // its remaining candidates become tasks; the owner then never backtracks
 // into a frozen level but claims its remaining tasks like any other wave.
 // Returns the number of nodes this wave evaluated.
-__device__ uint64_t run_dfs(const KArgs &a, const Sub &s, uint64_t *stk, Acc &acc, unsigned long long *s_hist,
-                            uint32_t me, bool &published) {
+template <class BmP, class GpP, class StP>
+__device__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm, GpP gP, StP stk, uint64_t *stk_g, Acc &acc,
+                            unsigned long long *s_hist, uint32_t me, bool &published, bool eager = false) {
+  constexpr bool kStkLds = !std::is_same<StP, uint64_t *>::value;
   const uint32_t W = s.W, p = s.p, rp = s.rows_pad, LS = s.LS;
   const uint32_t l = lane();
   const bool wl = l < W;
-  const uint64_t *bm = s.bm;
   const bool sharing = a.share_bound > 0;
   ShareRec *my = a.recs + me;
   uint64_t nodes = 0;
@@ -376,25 +387,38 @@ __device__ uint64_t run_dfs(const KArgs &a, const Sub &s, uint64_t *stk, Acc &ac
   int S = -1, Lstar = -1;  // levels <= S are frozen; Lstar: deepest level worth exposing
   uint32_t tick = 0;
   for (;;) {
-    uint64_t *rec = stk + (uint64_t)level * LS;
-    if (sharing && S < Lstar && S + 1 < level && ((++tick & 31) == 0)) {
+    StP rec = stk + (uint64_t)level * LS;
+    // An eager root (split before launch) exposes its top level at once;
+    // otherwise exposure waits for idle waves and is polled every 32 steps.
+    const bool eager_first = eager && S < 0;
+    if (sharing && S < Lstar && S + 1 < level && (eager_first || ((++tick & 31) == 0))) {
       bool go = false;
-      if (l == 0) go = ld_relaxed(a.idle) > 0 && ld_relaxed(&my->pending) == 0;
+      if (l == 0) go = (eager_first || ld_relaxed(a.idle) > 0) && ld_relaxed(&my->pending) == 0;
       if (bcast32(go)) {
         const uint32_t lp = (uint32_t)(S + 1);
-        const uint64_t *src = stk + (uint64_t)lp * LS;
+        StP src = stk + (uint64_t)lp * LS;
         uint32_t n = wave_sum_u32(wl ? (uint32_t)__popcll(src[3 * W + l]) : 0u);
-        if (n > 0 && l == 0) {
-          __hip_atomic_store(&my->pending, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          my->snap = src;
-          my->sub = s;
-          uint32_t ep = my->epoch + 1;
-          my->epoch = ep;
-          __hip_atomic_store(&my->ctl, ((unsigned long long)ep << 32) | ((unsigned long long)n << 16),
-                             __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-          publish_bit(a, me, true);
+        if (n > 0) {
+          const uint64_t *snap_g;
+          if (kStkLds) {
+            uint64_t *dst = stk_g + (uint64_t)lp * LS;
+            for (uint32_t i = l; i < LS; i += 64) dst[i] = src[i];
+            snap_g = dst;
+          } else {
+            snap_g = (const uint64_t *)src;
+          }
+          if (l == 0) {
+            __hip_atomic_store(&my->pending, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            my->snap = snap_g;
+            my->sub = s;
+            uint32_t ep = my->epoch + 1;
+            my->epoch = ep;
+            __hip_atomic_store(&my->ctl, ((unsigned long long)ep << 32) | ((unsigned long long)n << 16),
+                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            publish_bit(a, me, true);
+          }
+          published = true;
         }
-        if (n > 0) published = true;
         S = (int)lp;
       }
     }
@@ -420,7 +444,7 @@ __device__ uint64_t run_dfs(const KArgs &a, const Sub &s, uint64_t *stk, Acc &ac
           best = kk < best ? kk : best;
         }
       }
-      const uint64_t *XX = rec + 4 * W;
+      StP XX = rec + 4 * W;
       for (uint32_t j = L.xlo; j < L.xhi; ++j) {
         uint64_t cw = XX[j];
         if (cw == 0) continue;
@@ -442,7 +466,7 @@ __device__ uint64_t run_dfs(const KArgs &a, const Sub &s, uint64_t *stk, Acc &ac
       L.vk = wl ? (L.pk & ~bm[(uint64_t)l * rp + piv]) : 0;
       acc.words += W;
       if (wl) rec[3 * W + l] = L.vk;
-      if (sharing && (int)psz >= a.share_bound) Lstar = level;
+      if (sharing && ((int)psz >= a.share_bound || (eager && level == 0))) Lstar = level;
       st = 1;
       continue;
     }
@@ -463,7 +487,7 @@ __device__ uint64_t run_dfs(const KArgs &a, const Sub &s, uint64_t *stk, Acc &ac
         rec[2 * W + l] = L.xpk;
         rec[3 * W + l] = L.vk;
       }
-      L = make_child(a, s, L, rec + 4 * W, kc * 64 + b, rec + LS, acc);
+      L = make_child(s, L, rec + 4 * W, kc * 64 + b, rec + LS, bm, gP, acc);
       ++level;
       st = 0;
       continue;
@@ -477,16 +501,15 @@ __device__ uint64_t run_dfs(const KArgs &a, const Sub &s, uint64_t *stk, Acc &ac
       idx = (int)bcast32((uint32_t)idx);
       if (idx >= 0) {
         if (bcast32(last) && l == 0) publish_bit(a, me, false);
-        const uint64_t *snap = stk + (uint64_t)S * LS;
         level = S + 1;
-        L = materialize(a, s, snap, (uint32_t)idx, stk + (uint64_t)level * LS, acc);
+        L = materialize(s, stk + (uint64_t)S * LS, (uint32_t)idx, stk + (uint64_t)level * LS, bm, gP, acc);
         if (l == 0) __hip_atomic_fetch_sub(&my->pending, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         Lstar = Lstar < S ? Lstar : S;
         st = 0;
         continue;
       }
       // All tasks taken; wait until the takers have copied their part of the
-      // frozen levels out of this stack.
+      // frozen levels out.
       if (l == 0) {
         uint32_t n = 1;
         while (ld_acquire(&my->pending) != 0) {
@@ -516,10 +539,48 @@ __device__ void defer_root(const KArgs &a, uint64_t canon, uint64_t data_need, u
   }
 }
 
+// LDS words a subproblem needs for the LDS tier: bitmap, label hashes,
+// `levels` stack records.
+__device__ __forceinline__ uint64_t lds_words(const Sub &s) {
+  return (uint64_t)s.W * s.rows_pad + s.p + (uint64_t)s.levels * s.LS;
+}
+
+// Copies bitmap and label hashes of s from HBM into the LDS tier layout
+// [bm | gP | stack]; returns the stack base.
+__device__ __forceinline__ lds64 *stage_lds(const Sub &s, lds64 *lds) {
+  const uint32_t l = lane();
+  const uint64_t nb = (uint64_t)s.W * s.rows_pad;
+  for (uint64_t i = l; i < nb; i += 64) lds[i] = s.bm[i];
+  for (uint64_t i = l; i < s.p; i += 64) lds[nb + i] = s.gP[i];
+  return lds + nb + s.p;
+}
+
+// Batches: moves what this wave accumulated since the last flush (count,
+// hash, LDS histogram) to graph g's results.
+__device__ __forceinline__ void flush_graph(const KArgs &a, uint32_t g, Acc &acc, unsigned long long *s_hist) {
+  const uint32_t l = lane();
+  if (l == 0) {
+    if (acc.count) atomicAdd(&a.g_count[g], acc.count);
+    if (acc.hash) atomicAdd(&a.g_hash[g], acc.hash);
+    acc.count = 0;
+    acc.hash = 0;
+  }
+  for (uint32_t i = l; i < kLdsHist; i += 64) {
+    unsigned long long v = s_hist[i];
+    if (v) {
+      if (i < a.g_hist_stride) atomicAdd(&a.g_hist[(uint64_t)g * a.g_hist_stride + i], v);
+      else atomicOr(a.err, kErrHistOverflow);
+      s_hist[i] = 0;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Builds root `canon` in this wave's data region and searches it.
 // Returns tree nodes evaluated by this wave.
-__device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk, uint32_t me, uint64_t canon, Acc &acc,
-                               unsigned long long *s_hist, bool &published) {
+__device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk_g, lds64 *lds, uint32_t me,
+                               uint64_t canon, Acc &acc, unsigned long long *s_hist, bool &published,
+                               bool eager) {
   const uint32_t l = lane();
   const uint64_t desc = a.roots[canon];
   uint32_t *C = (uint32_t *)slot;
@@ -527,6 +588,7 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk, uin
   Sub s;
   s.canon = canon;
   s.bm_owner = me;
+  s.graph = a.root_graph ? a.root_graph[canon] : 0;
   uint64_t c = 0, x = 0;
   if (a.root_mode == 2) {
     s.v0 = (uint32_t)(desc >> 32);
@@ -589,7 +651,7 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk, uin
   }
   const uint64_t W = (p + 63) / 64;
   if (c > ccap) {  // candidate list alone exceeds the data region
-    defer_root(a, canon, align_up(4 * c, 256) + align_up(8 * W * align_up(c, 64), 256),
+    defer_root(a, canon, align_up(4 * c, 256) + align_up(8 * W * align_up(c, 64), 256) + align_up(8 * p, 256),
                (p + 1) * (4 * W + (x + 63) / 64 + 4) * 8);
     return 0;
   }
@@ -618,16 +680,19 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk, uin
   const uint64_t rows = p + xq;
   const uint32_t rows_pad = (uint32_t)align_up(rows, 64);
   const uint64_t bm_off = align_up(4 * c, 256);
-  const uint64_t data_need = bm_off + align_up(8ull * W * rows_pad, 256);
+  const uint64_t gp_off = bm_off + align_up(8ull * W * rows_pad, 256);
+  const uint64_t data_need = gp_off + align_up(8 * p, 256);
   const uint32_t LS = (uint32_t)(4 * W + Wx + 4);
   if (data_need > a.slot_bytes) {
     defer_root(a, canon, data_need, (p + 1) * LS * 8);
     return 0;
   }
   uint64_t *bm = (uint64_t *)(slot + bm_off);
+  uint64_t *gP = (uint64_t *)(slot + gp_off);
   acc.build += 4 * c + 8 * W * rows_pad;
   // Bitmap (word-major): row i is P0[i] for i < p, else X0'[i - p].
   for (uint64_t i = l; i < W * rows_pad; i += 64) bm[i] = 0;
+  for (uint64_t i = l; i < p; i += 64) gP[i] = a.label_hash[Plist[i]];
   wave_fence();
   uint32_t maxdeg = 0;
   for (uint64_t i = l; i < rows; i += 64) {
@@ -649,6 +714,7 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk, uin
     if (i < p) maxdeg = deg > maxdeg ? deg : maxdeg;
   }
   maxdeg = wave_max_u32(maxdeg);
+  wave_fence();
   // Depth bound: a clique inside P0 has at most maxdeg + 1 vertices.
   const uint64_t levels = (p < (uint64_t)maxdeg + 1 ? p : (uint64_t)maxdeg + 1) + 1;
   if (levels * LS * 8 > a.stack_bytes) {
@@ -661,8 +727,10 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk, uin
   s.Wx = Wx;
   s.rows_pad = rows_pad;
   s.LS = LS;
+  s.levels = (uint32_t)levels;
   s.Plist = Plist;
   s.bm = bm;
+  s.gP = gP;
   // Level 0: R = {}, P = P0, XP = {}, XX = X0'.
   Lvl L0;
   L0.rk = 0;
@@ -674,16 +742,25 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk, uin
   L0.xxcnt = (uint32_t)xq;
   L0.xlo = 0;
   L0.xhi = Wx;
-  for (uint32_t j = l; j < Wx; j += 64) stk[4 * W + j] = low_mask(xq - (uint64_t)j * 64);
-  store_rpx(stk, s, L0);
+  if (a.lds_tier && lds_words(s) * 8 <= a.lds_bytes) {
+    lds64 *st = stage_lds(s, lds);
+    for (uint32_t j = l; j < Wx; j += 64) st[4 * W + j] = low_mask(xq - (uint64_t)j * 64);
+    store_rpx(st, s, L0);
+    wave_fence();
+    return run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + W * rows_pad), st, stk_g, acc, s_hist, me,
+                   published, eager);
+  }
+  for (uint32_t j = l; j < Wx; j += 64) stk_g[4 * W + j] = low_mask(xq - (uint64_t)j * 64);
+  store_rpx(stk_g, s, L0);
   wave_fence();
-  return run_dfs(a, s, stk, acc, s_hist, me, published);
+  return run_dfs(a, s, (const uint64_t *)bm, (const uint64_t *)gP, stk_g, stk_g, acc, s_hist, me, published,
+                 eager);
 }
 
 // Looks for a published task (preferring waves on this XCD, which share
 // this wave's L2) and runs it.  Returns true when a task was run.
-__device__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk, Acc &acc, unsigned long long *s_hist,
-                      bool &is_idle) {
+__device__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk_g, lds64 *lds, Acc &acc,
+                      unsigned long long *s_hist, bool &is_idle) {
   const uint32_t l = lane();
   const uint32_t nwords = (a.n_waves + 63) / 64;
   const uint64_t xcd = 0x0101010101010101ull << (me & 7);
@@ -724,11 +801,26 @@ __device__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk, Acc &acc, unsi
   Sub s = r->sub;
   const uint64_t *snap = r->snap;
   if (l == 0) atomicAdd(&a.users[s.bm_owner], 1u);
-  materialize(a, s, snap, (uint32_t)idx, stk, acc);
+  const bool in_lds = a.lds_tier && lds_words(s) * 8 <= a.lds_bytes;
+  lds64 *st = nullptr;
+  if (in_lds) {
+    st = stage_lds(s, lds);
+    wave_fence();
+    materialize(s, snap, (uint32_t)idx, st, (const lds64 *)lds, (const lds64 *)(lds + (uint64_t)s.W * s.rows_pad),
+                acc);
+  } else {
+    materialize(s, snap, (uint32_t)idx, stk_g, s.bm, s.gP, acc);
+  }
   if (l == 0) __hip_atomic_fetch_sub(&r->pending, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   uint64_t t0 = clock64();
   bool pub_unused = false;
-  uint64_t nodes = run_dfs(a, s, stk, acc, s_hist, me, pub_unused);
+  uint64_t nodes;
+  if (in_lds)
+    nodes = run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + (uint64_t)s.W * s.rows_pad), st, stk_g, acc,
+                    s_hist, me, pub_unused);
+  else
+    nodes = run_dfs(a, s, s.bm, s.gP, stk_g, stk_g, acc, s_hist, me, pub_unused);
+  if (a.root_graph) flush_graph(a, s.graph, acc, s_hist);
   if (l == 0) {
     if (a.root_nodes && nodes) atomicAdd(&a.root_nodes[s.canon], (unsigned long long)nodes);
     if (a.root_cycles) atomicAdd(&a.root_cycles[s.canon], (unsigned long long)(clock64() - t0));
@@ -740,11 +832,14 @@ __device__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk, Acc &acc, unsi
   return true;
 }
 
+extern __shared__ uint64_t lds_dyn[];
+
 __global__ __launch_bounds__(64) void mce_kernel(KArgs a) {
   __shared__ unsigned long long s_hist[kLdsHist];
   const uint32_t l = lane();
   for (uint32_t i = l; i < kLdsHist; i += 64) s_hist[i] = 0;
   __syncthreads();
+  lds64 *lds = (lds64 *)lds_dyn;
   const uint32_t me = blockIdx.x;
   uint8_t *slot = a.slots + (uint64_t)me * a.slot_bytes;
   uint64_t *stk = (uint64_t *)(a.stacks + (uint64_t)me * a.stack_bytes);
@@ -777,8 +872,10 @@ __global__ __launch_bounds__(64) void mce_kernel(KArgs a) {
           is_idle = false;
           uint64_t canon = a.perm ? a.perm[i] : (a.reverse ? a.n_dispatch - 1 - i : i);
           uint64_t t0 = clock64();
-          uint64_t nodes = solve_root(a, slot, stk, me, canon, acc, s_hist, published);
+          const bool eager = a.eager && a.eager[i];
+          uint64_t nodes = solve_root(a, slot, stk, lds, me, canon, acc, s_hist, published, eager);
           nodes_total += nodes;
+          if (a.root_graph) flush_graph(a, a.root_graph[canon], acc, s_hist);
           if (l == 0) {
             if (a.root_nodes && nodes) atomicAdd(&a.root_nodes[canon], (unsigned long long)nodes);
             if (a.root_cycles) atomicAdd(&a.root_cycles[canon], (unsigned long long)(clock64() - t0));
@@ -796,7 +893,7 @@ __global__ __launch_bounds__(64) void mce_kernel(KArgs a) {
       if (l == 0) atomicAdd(a.idle, 1u);
       is_idle = true;
     }
-    if (a.share_bound > 0 && steal(a, me, stk, acc, s_hist, is_idle)) {
+    if (a.share_bound > 0 && steal(a, me, stk, lds, acc, s_hist, is_idle)) {
       backoff = 1;
       continue;
     }
@@ -821,7 +918,7 @@ __global__ __launch_bounds__(64) void mce_kernel(KArgs a) {
 }  // namespace
 
 hipError_t launch_mce(const KArgs &a, uint32_t waves, hipStream_t stream) {
-  hipLaunchKernelGGL(mce_kernel, dim3(waves), dim3(64), 0, stream, a);
+  hipLaunchKernelGGL(mce_kernel, dim3(waves), dim3(64), a.lds_tier ? a.lds_bytes : 0, stream, a);
   return hipGetLastError();
 }
 
@@ -837,6 +934,20 @@ __global__ __launch_bounds__(256) void copy_probe_kernel(const uint4 *__restrict
     dst[i] = src[i];
 }
 }  // namespace
+
+namespace {
+__global__ __launch_bounds__(256) void fill_zero_kernel(uint64_t *p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+}  // namespace
+
+hipError_t launch_fill_zero(void *p, uint64_t n_words, hipStream_t stream) {
+  if (n_words == 0) return hipSuccess;
+  uint32_t blocks = (uint32_t)std::min<uint64_t>(2048, (n_words + 255) / 256);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, stream, (uint64_t *)p, n_words);
+  return hipGetLastError();
+}
 
 hipError_t launch_copy_probe(const void *src, void *dst, uint64_t bytes, hipStream_t stream) {
   uint64_t n = bytes / 16;

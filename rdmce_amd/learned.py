@@ -1,0 +1,178 @@
+"""Learned root scheduling.
+
+An FP32 MLP (8 -> 32 -> 32 -> 1, ReLU; HIP kernels in librdmce.so) predicts
+log(1 + cycles) of every edge root from structural features computed on the
+GPU.  Labels come from a plain run with per-root statistics (the cycles the
+waves spent on each root, summed over waves when it was shared).  The
+schedule dispatches roots in decreasing predicted cost (longest first) and
+flags the predicted heaviest for an eager split: their top level is exposed
+to other waves as soon as it is known.  Ordering and splitting never change
+results; with a model that predicts garbage (non-finite outputs) the plain
+order is used.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+from ctypes import POINTER, byref, c_float, c_uint64, c_void_p
+
+import numpy as np
+
+from ._lib import check, lib
+
+NFEAT = 8
+
+
+def _fptr(a):
+    return a.ctypes.data_as(POINTER(c_float))
+
+
+def _setup():
+    L = lib()
+    if getattr(L, "_model_sigs", False):
+        return L
+    G = c_void_p
+    sigs = {
+        "mce_root_features": ([G, ctypes.c_int32, POINTER(c_float), c_uint64, POINTER(c_uint64)], ctypes.c_int32),
+        "mce_cost_model_num_params": ([], ctypes.c_uint32),
+        "mce_cost_model_num_trainable": ([], ctypes.c_uint32),
+        "mce_cost_model_create": ([c_uint64, POINTER(c_void_p)], ctypes.c_int32),
+        "mce_cost_model_get_params": ([c_void_p, POINTER(c_float), c_uint64], ctypes.c_int32),
+        "mce_cost_model_set_params": ([c_void_p, POINTER(c_float), c_uint64], ctypes.c_int32),
+        "mce_cost_model_fit_normalizer": ([c_void_p, POINTER(c_float), c_uint64], ctypes.c_int32),
+        "mce_cost_model_gradient": ([c_void_p, ctypes.c_int32, POINTER(c_float), POINTER(c_float), c_uint64,
+                                     POINTER(c_float), POINTER(c_float)], ctypes.c_int32),
+        "mce_cost_model_train": ([c_void_p, ctypes.c_int32, POINTER(c_float), POINTER(c_float), c_uint64,
+                                  ctypes.c_uint32, ctypes.c_uint32, c_float, c_uint64, POINTER(c_float)],
+                                 ctypes.c_int32),
+        "mce_cost_model_predict": ([c_void_p, ctypes.c_int32, POINTER(c_float), c_uint64, POINTER(c_float)],
+                                   ctypes.c_int32),
+        "mce_cost_model_free": ([c_void_p], None),
+    }
+    for name, (args, ret) in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ret
+    L._model_sigs = True
+    return L
+
+
+def root_features(graph, device: int = 0) -> np.ndarray:
+    """[m, 8] float32 features of the canonical edge roots (GPU)."""
+    L = _setup()
+    n = c_uint64()
+    check(L.mce_root_features(graph.handle, device, None, 0, byref(n)))
+    out = np.zeros((max(n.value, 1), NFEAT), dtype=np.float32)
+    check(L.mce_root_features(graph.handle, device, _fptr(out), out.size, byref(n)))
+    return out[:n.value]
+
+
+class CostModel:
+    def __init__(self, seed: int = 0):
+        self.L = _setup()
+        self._h = c_void_p()
+        check(self.L.mce_cost_model_create(seed, byref(self._h)))
+
+    @property
+    def n_params(self) -> int:
+        return int(self.L.mce_cost_model_num_params())
+
+    @property
+    def n_trainable(self) -> int:
+        return int(self.L.mce_cost_model_num_trainable())
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, dtype=np.float32)
+        check(self.L.mce_cost_model_get_params(self._h, _fptr(p), p.size))
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(self.L.mce_cost_model_set_params(self._h, _fptr(p), p.size))
+
+    def fit_normalizer(self, X) -> None:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        check(self.L.mce_cost_model_fit_normalizer(self._h, _fptr(X), X.shape[0]))
+
+    def gradient(self, X, Y, device: int = 0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        g = np.zeros(self.n_trainable, dtype=np.float32)
+        loss = c_float()
+        check(self.L.mce_cost_model_gradient(self._h, device, _fptr(X), _fptr(Y), X.shape[0], _fptr(g), byref(loss)))
+        return g, float(loss.value)
+
+    def train(self, X, Y, epochs: int = 20, batch: int = 1024, lr: float = 3e-3, seed: int = 0,
+              device: int = 0) -> float:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        loss = c_float()
+        check(self.L.mce_cost_model_train(self._h, device, _fptr(X), _fptr(Y), X.shape[0], epochs, batch, lr, seed,
+                                          byref(loss)))
+        return float(loss.value)
+
+    def predict(self, X, device: int = 0) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros(max(X.shape[0], 1), dtype=np.float32)
+        check(self.L.mce_cost_model_predict(self._h, device, _fptr(X), X.shape[0], _fptr(out)))
+        return out[:X.shape[0]]
+
+    def free(self):
+        if getattr(self, "_h", None):
+            self.L.mce_cost_model_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.free()
+
+
+@dataclasses.dataclass
+class Schedule:
+    order: np.ndarray      # canonical root ids in dispatch order
+    eager: np.ndarray      # uint8 per dispatch position
+    learned: bool          # False: fell back to the plain order
+
+
+def plain_schedule(root_ids) -> Schedule:
+    """Plain method: roots in decreasing canonical id (high core first),
+    BkPivotBitBalance.cuh:429-431."""
+    ids = np.sort(np.asarray(root_ids, dtype=np.uint64))[::-1].copy()
+    return Schedule(ids, np.zeros(ids.size, dtype=np.uint8), False)
+
+
+def learned_schedule(pred, root_ids, eager_top: int = 0) -> Schedule:
+    """Longest-predicted-first order of `root_ids` (canonical ids) from the
+    per-root predictions `pred` (indexed by canonical id); the `eager_top`
+    heaviest are split at launch.  Falls back to the plain order when a
+    prediction is not finite."""
+    root_ids = np.asarray(root_ids, dtype=np.uint64)
+    p = np.asarray(pred, dtype=np.float64)[root_ids.astype(np.int64)]
+    if p.size and not np.all(np.isfinite(p)):
+        return plain_schedule(root_ids)
+    # stable: equal predictions keep the plain (decreasing id) order
+    base = np.sort(root_ids)[::-1]
+    pb = np.asarray(pred, dtype=np.float64)[base.astype(np.int64)]
+    order = base[np.argsort(-pb, kind="stable")].copy()
+    eager = np.zeros(order.size, dtype=np.uint8)
+    eager[:min(eager_top, order.size)] = 1
+    return Schedule(order, eager, True)
+
+
+def training_set(features, root_cycles, root_ids, max_samples: int = 1 << 20, seed: int = 0):
+    """(X, Y) from a plain run's per-root cycles: Y = log1p(cycles)."""
+    rng = np.random.default_rng(seed)
+    ids = np.asarray(root_ids, dtype=np.int64)
+    if ids.size > max_samples:
+        ids = rng.choice(ids, size=max_samples, replace=False)
+    X = np.ascontiguousarray(features[ids], dtype=np.float32)
+    Y = np.log1p(np.asarray(root_cycles, dtype=np.float64)[ids]).astype(np.float32)
+    return X, Y
+
+
+def spearman(a, b) -> float:
+    ra = np.argsort(np.argsort(a, kind="stable"), kind="stable").astype(np.float64)
+    rb = np.argsort(np.argsort(b, kind="stable"), kind="stable").astype(np.float64)
+    ra -= ra.mean()
+    rb -= rb.mean()
+    d = np.sqrt((ra * ra).sum() * (rb * rb).sum())
+    return float((ra * rb).sum() / d) if d else 0.0

@@ -12,6 +12,7 @@ statistics.  All work runs in librdmce.so's HIP kernels; there is no CPU path.
 """
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import time
 from ctypes import POINTER, byref, c_uint64, c_void_p
@@ -47,7 +48,8 @@ class MceOutput:
 
 def make_options(device: int = 0, root_mode: int = _lib.ROOT_EDGE, waves_per_cu: int = 0, share_bound: int = 0,
                  want_list: bool = False, want_root_stats: bool = False, list_capacity: int = 0,
-                 slot_bytes: int = 0, root_order=None, lds_tier: int = 1) -> tuple[MceOptions, object]:
+                 slot_bytes: int = 0, root_order=None, lds_tier: int = 1, use_graph: bool = False,
+                 lds_bytes: int = 0, eager=None) -> tuple[MceOptions, object]:
     o = MceOptions()
     lib().mce_options_init(byref(o))
     o.device = device
@@ -59,12 +61,19 @@ def make_options(device: int = 0, root_mode: int = _lib.ROOT_EDGE, waves_per_cu:
     o.list_capacity = list_capacity
     o.slot_bytes = slot_bytes
     o.lds_tier = lds_tier
-    keep = None
+    o.use_graph = int(use_graph)
+    o.lds_bytes = lds_bytes
+    keep = []
     if root_order is not None:
-        keep = np.ascontiguousarray(root_order, dtype=np.uint64)
+        ro = np.ascontiguousarray(root_order, dtype=np.uint64)
+        keep.append(ro)
         o.schedule = _lib.SCHED_CUSTOM
-        o.roots = keep.ctypes.data_as(POINTER(c_uint64))
-        o.n_roots = keep.size
+        o.roots = ro.ctypes.data_as(POINTER(c_uint64))
+        o.n_roots = ro.size
+    if eager is not None:
+        ea = np.ascontiguousarray(eager, dtype=np.uint8)
+        keep.append(ea)
+        o.eager = ea.ctypes.data_as(POINTER(ctypes.c_uint8))
     return o, keep
 
 
@@ -116,9 +125,75 @@ class Plan:
         finally:
             lib().mce_result_free(res)
 
+    def autotune(self, stride: int = 1) -> dict:
+        """Picks waves/CU, LDS tier and sharing bound on a sample of the roots
+        (every `stride`-th); later runs use the winner."""
+        t = _lib.MceTuneResult()
+        check(lib().mce_plan_autotune(self._h, stride, byref(t)))
+        return _tune_dict(t)
+
     def free(self) -> None:
         if getattr(self, "_h", None):
             lib().mce_plan_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.free()
+
+
+def _tune_dict(t) -> dict:
+    return {"waves_per_cu": t.waves_per_cu, "share_bound": t.share_bound, "lds_tier": t.lds_tier,
+            "trials": t.trials, "best_ms": t.best_ms, "base_ms": t.base_ms}
+
+
+@dataclasses.dataclass
+class BatchOutput:
+    count: np.ndarray           # per graph
+    hash: np.ndarray            # per graph
+    hist: np.ndarray            # [graph, size]
+    total_count: int
+    total_nodes: int
+    kernel_ms: float
+    total_ms: float
+    graph_replay: bool
+    passes: int
+
+    def hist_dict(self, g: int) -> dict:
+        return {int(k): int(v) for k, v in enumerate(self.hist[g]) if v}
+
+
+class Batch:
+    """Many independent graphs enumerated by one persistent launch
+    (optionally replayed as a captured hipGraph: use_graph=True)."""
+
+    def __init__(self, graphs, **kw):
+        self._o, self._keep = make_options(**kw)
+        arr = (c_void_p * len(graphs))(*[g.handle.value for g in graphs])
+        self._h = c_void_p()
+        check(lib().mce_batch_create(arr, len(graphs), byref(self._o), byref(self._h)))
+
+    def run(self) -> BatchOutput:
+        res = POINTER(_lib.MceBatchResult)()
+        check(lib().mce_batch_run(self._h, byref(res)))
+        try:
+            r = res.contents
+            n = int(r.n_graphs)
+            cnt = np.ctypeslib.as_array(r.count, shape=(n,)).copy()
+            hs = np.ctypeslib.as_array(r.hash, shape=(n,)).copy()
+            hist = np.ctypeslib.as_array(r.hist, shape=(n * r.hist_stride,)).copy().reshape(n, r.hist_stride)
+            return BatchOutput(cnt, hs, hist, int(r.total_count), int(r.total_nodes), float(r.kernel_ms),
+                               float(r.total_ms), bool(r.graph_replay), int(r.passes))
+        finally:
+            lib().mce_batch_result_free(res)
+
+    def autotune(self, stride: int = 1) -> dict:
+        t = _lib.MceTuneResult()
+        check(lib().mce_batch_autotune(self._h, stride, byref(t)))
+        return _tune_dict(t)
+
+    def free(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mce_batch_free(self._h)
             self._h = None
 
     def __del__(self):

@@ -1,0 +1,106 @@
+"""Learned cost model on the GPU: HIP forward/backward against the NumPy
+reference (FP32 tolerances stated below), GPU features against a NumPy
+restatement, training, and the schedule's invariance of results."""
+import sys
+
+import numpy as np
+import pytest
+
+import mlp_ref
+import rdmce_amd as R
+from graphs import random_graph
+from rdmce_amd import learned
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _model_and_data(seed=0, n=3000):
+    rng = np.random.default_rng(seed)
+    m = learned.CostModel(seed)
+    X = rng.normal(0, 1, (n, 8)).astype(np.float32) * np.array([1, 2, 0.3, 0.3, 1, 1, 2, 4], np.float32)
+    Y = (X[:, 6] * 0.7 + np.abs(X[:, 1]) * 0.2 + rng.normal(0, 0.1, n)).astype(np.float32)
+    m.fit_normalizer(X)
+    return m, X, Y
+
+
+def test_forward_matches_reference():
+    banner("mlp forward")
+    m, X, Y = _model_and_data()
+    got = m.predict(X)
+    ref, _ = mlp_ref.forward(m.params(), X)
+    # FP32 against float64: |err| <= 1e-4 * (1 + |y|)
+    assert np.all(np.abs(got - ref) <= 1e-4 * (1 + np.abs(ref)))
+
+
+def test_gradient_matches_reference():
+    banner("mlp gradient")
+    m, X, Y = _model_and_data(1)
+    g, loss = m.gradient(X, Y)
+    p = m.params()
+    gr = mlp_ref.gradient(p, X, Y)
+    assert abs(loss - mlp_ref.loss(p, X, Y)) <= 1e-4 * (1 + abs(loss))
+    # FP32 sums over 3000 samples: relative 1e-3 of the gradient norm
+    assert np.max(np.abs(g - gr)) <= 1e-3 * (np.max(np.abs(gr)) + 1e-6)
+    g2, _ = m.gradient(X, Y)
+    assert np.array_equal(g, g2)  # deterministic reduction order
+
+
+def test_training_reduces_loss():
+    banner("mlp training")
+    m, X, Y = _model_and_data(2, 20000)
+    p0 = m.params()
+    l0 = mlp_ref.loss(p0, X, Y)
+    lf = m.train(X, Y, epochs=15, batch=512, lr=3e-3, seed=3)
+    l1 = mlp_ref.loss(m.params(), X, Y)
+    assert l1 < 0.25 * l0 and abs(lf - l1) < 0.5 * l1 + 1e-3
+
+
+def test_features_match_numpy():
+    banner("root features")
+    g = R.Graph.from_edges(random_graph(120, 0.3, 5))
+    g.SortByOrder(R.OrderType.DEG)
+    F = learned.root_features(g)
+    ro, co = g.GetRowOffset().astype(np.int64), g.GetColIdx().astype(np.int64)
+    core = g.GetCoreNumbers().astype(np.float64)
+    d = g.GetDegeneracy()
+    roots = g.roots()
+    assert F.shape == (roots.size, 8)
+    for r, desc in enumerate(roots[:400]):
+        v0, v1 = int(desc >> np.uint64(32)), int(desc & np.uint64(0xFFFFFFFF))
+        n0, n1 = set(co[ro[v0]:ro[v0 + 1]]), co[ro[v1]:ro[v1 + 1]]
+        t = sum(1 for w in n1 if w > v1 and w in n0)
+        exp = [np.log1p(len(n0)), np.log1p(len(n1)), core[v0] / d, core[v1] / d,
+               np.log1p(sum(1 for w in n0 if w > v0)), np.log1p(sum(1 for w in n1 if w > v1)),
+               np.log1p(t), np.log1p(t) ** 2]
+        assert np.allclose(F[r], exp, rtol=1e-6, atol=1e-6), r
+
+
+def test_learned_schedule_keeps_results_and_fallback():
+    banner("learned schedule invariance")
+    g = R.Graph.from_edges(random_graph(300, 0.45, 9))
+    g.SortByOrder(R.OrderType.DEG)
+    base = R.solve(g, want_root_stats=True)
+    F = learned.root_features(g)
+    ids = np.arange(g.GetNumEdges(), dtype=np.uint64)
+    X, Y = learned.training_set(F, base.root_cycles, ids)
+    m = learned.CostModel(4)
+    m.fit_normalizer(X)
+    m.train(X, Y, epochs=10, batch=256)
+    pred = m.predict(F)
+    s = learned.learned_schedule(pred, ids, eager_top=20)
+    assert s.learned
+    out = R.solve(g, root_order=s.order, eager=s.eager, want_root_stats=True)
+    assert (out.count, out.hash, out.hist_dict()) == (base.count, base.hash, base.hist_dict())
+    assert np.array_equal(out.root_nodes, base.root_nodes)
+    bad = m.params()
+    bad[m.n_trainable - 1] = np.nan  # output bias: every prediction becomes NaN
+    m.set_params(bad)
+    s2 = learned.learned_schedule(m.predict(F), ids, eager_top=20)
+    assert not s2.learned
+    out2 = R.solve(g, root_order=s2.order, eager=s2.eager)
+    assert (out2.count, out2.hash) == (base.count, base.hash)

@@ -167,3 +167,96 @@ def test_sharing_single_heavy_root():
     ref = oracle_for(g, root_nodes=True)
     check_same(out, ref, nodes=True)
     assert out.shared_tasks > 0
+
+
+def test_rmat20_core499_full_size_golden():
+    """The bench workload at full size against the committed CPU-oracle
+    golden (tests/golden/make_rmat_golden.py): 13.2M roots, ~1.0e9 cliques."""
+    import json
+    banner("rmat20 core<=499 full size")
+    with open(os.path.join(GOLDEN, "rmat20_s2_core499.json")) as f:
+        gd = json.load(f)
+    g = R.Graph.rmat(20, 16, 0.57, 0.19, 0.19, seed=2)
+    g.SortByOrder(R.OrderType.DEG)
+    core = g.GetCoreNumbers()
+    roots = g.roots(R.ROOT_EDGE)
+    v0 = (roots >> np.uint64(32)).astype(np.int64)
+    end = int(np.searchsorted(core[v0], gd["core_max"], side="right"))
+    assert end == gd["roots"]
+    out = R.solve(g, root_order=np.arange(end, dtype=np.uint64)[::-1].copy())
+    assert out.count == gd["count"]
+    assert out.hash == gd["hash"]
+    assert out.hist_dict() == {int(k): v for k, v in gd["hist"].items()}
+    assert out.total_nodes == gd["tree_nodes"]
+
+
+def test_batch_per_graph_results():
+    """Many graphs in one launch: per-graph count, hash and histogram equal
+    each graph's own oracle result."""
+    banner("batch per-graph")
+    graphs, refs = [], []
+    for i in range(40):
+        n, p, s = R.batch_graph_params(3, i, 32, 160, 0.05, 0.5)
+        g = R.Graph.erdos_renyi(n, p, s)
+        g.SortByOrder(R.OrderType.DEG)
+        graphs.append(g)
+        refs.append(oracle_for(g))
+    b = R.Batch(graphs)
+    out = b.run()
+    b.free()
+    for i, ref in enumerate(refs):
+        assert int(out.count[i]) == ref.count, i
+        assert int(out.hash[i]) == ref.hash, i
+        assert out.hist_dict(i) == ref.hist, i
+    assert out.total_count == sum(r.count for r in refs)
+
+
+def test_batch_graph_replay_matches_eager():
+    """Captured hipGraph replays (reset fills + kernel) equal the eager run,
+    with eager work and synchronisation between replays."""
+    banner("batch graph replay")
+    graphs = []
+    for i in range(25):
+        n, p, s = R.batch_graph_params(3, 100 + i, 32, 200, 0.05, 0.5)
+        g = R.Graph.erdos_renyi(n, p, s)
+        g.SortByOrder(R.OrderType.DEG)
+        graphs.append(g)
+    eager = R.Batch(graphs).run()
+    b = R.Batch(graphs, use_graph=True)
+    for rep in range(4):
+        out = b.run()
+        assert out.graph_replay
+        assert np.array_equal(out.count, eager.count) and np.array_equal(out.hash, eager.hash)
+        assert np.array_equal(out.hist, eager.hist)
+        R.solve(graphs[rep])  # unrelated eager work between replays
+    b.free()
+
+
+def test_plan_graph_replay_and_eager_split():
+    banner("plan graph replay + eager split")
+    g = R.Graph.from_edges(random_graph(260, 0.5, 31))
+    g.SortByOrder(R.OrderType.DEG)
+    base = R.solve(g, want_root_stats=True)
+    m = g.GetNumEdges()
+    order = np.arange(m, dtype=np.uint64)[::-1].copy()
+    flags = np.zeros(m, dtype=np.uint8)
+    flags[: m // 10] = 1
+    plan = R.Plan(g, root_order=order, eager=flags, use_graph=True, want_root_stats=True)
+    for _ in range(3):
+        out = plan.run()
+        assert (out.count, out.hash, out.hist_dict()) == (base.count, base.hash, base.hist_dict())
+        assert np.array_equal(out.root_nodes, base.root_nodes)
+    plan.free()
+
+
+def test_autotune_keeps_results():
+    banner("autotune")
+    g = R.Graph.from_edges(random_graph(300, 0.45, 32))
+    g.SortByOrder(R.OrderType.DEG)
+    base = R.solve(g)
+    plan = R.Plan(g)
+    t = plan.autotune(stride=2)
+    assert t["trials"] >= 10 and t["best_ms"] <= t["base_ms"] * 1.0001
+    out = plan.run()
+    plan.free()
+    assert (out.count, out.hash, out.hist_dict()) == (base.count, base.hash, base.hist_dict())
