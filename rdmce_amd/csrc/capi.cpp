@@ -364,6 +364,7 @@ struct mce_plan {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   uint64_t *d_rowoff = nullptr;
+  uint64_t *d_outoff = nullptr;
   uint32_t *d_col = nullptr;
   uint64_t *d_lhash = nullptr;
   uint32_t *d_labels = nullptr;
@@ -423,7 +424,7 @@ void plan_release(mce_plan *p) {
   if (!p) return;
   (void)hipSetDevice(p->device);
   drop_graph(p);
-  void *ptrs[] = {p->d_rowoff,   p->d_col,         p->d_lhash,      p->d_labels,      p->d_roots,
+  void *ptrs[] = {p->d_rowoff,   p->d_outoff,  p->d_col,         p->d_lhash,      p->d_labels,      p->d_roots,
                   p->d_perm,     p->d_eager,       p->d_slots,      p->d_ctr,         p->d_hist,
                   p->d_root_nodes, p->d_root_cycles, p->d_list,     p->d_deferred,    p->d_deferred2,
                   p->d_slots2,   p->d_stacks,      p->d_stacks2,    p->d_recs,        p->d_users,
@@ -519,6 +520,18 @@ mce_status plan_create_impl(const PlanSpec &spec, const mce_options &opt, mce_pl
   for (uint64_t v = 0; v < g.n; ++v) lhash[v] = splitmix64_mix(g.labels[v]);
 
   PTRY(dalloc(&p->d_rowoff, g.n + 1));
+  {
+    std::vector<uint64_t> outoff(g.n + 1, 0);
+    parallel_chunks(g.n, [&](int, uint64_t b, uint64_t e) {
+      for (uint64_t v = b; v < e; ++v) {
+        const uint32_t *rb = g.col.data() + g.rowoff[v], *re = g.col.data() + g.rowoff[v + 1];
+        outoff[v] = g.rowoff[v] + (uint64_t)(std::upper_bound(rb, re, (uint32_t)v) - rb);
+      }
+    });
+    outoff[g.n] = g.rowoff[g.n];
+    PTRY(dalloc(&p->d_outoff, g.n + 1));
+    PTRY(hipMemcpy(p->d_outoff, outoff.data(), (g.n + 1) * 8, hipMemcpyHostToDevice));
+  }
   PTRY(dalloc(&p->d_col, g.col.size()));
   PTRY(dalloc(&p->d_lhash, g.n));
   PTRY(dalloc(&p->d_labels, g.n));
@@ -605,6 +618,7 @@ mce_status plan_create_impl(const PlanSpec &spec, const mce_options &opt, mce_pl
 KArgs make_args(const mce_plan *p) {
   KArgs a{};
   a.rowoff = p->d_rowoff;
+  a.outoff = p->d_outoff;
   a.col = p->d_col;
   a.label_hash = p->d_lhash;
   a.labels = p->d_labels;

@@ -43,6 +43,7 @@ struct alignas(128) ShareRec {
 struct KArgs {
   // Graph (CSR in HBM).
   const uint64_t *rowoff;
+  const uint64_t *outoff;       // per vertex: index in col of its first higher neighbour
   const uint32_t *col;
   const uint64_t *label_hash;   // splitmix64(label(v)) per vertex
   const uint32_t *labels;       // original ids (clique list)

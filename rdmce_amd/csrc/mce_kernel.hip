@@ -41,26 +41,34 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    uint64_t o = __shfl_xor(v, off, 64);
-    v = o < v ? o : v;
-  }
-  return v;
-}
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// Wave-wide reductions with DPP (row_shr 1/2/4/8 inside each 16-lane row,
+// then row_bcast 15/31 across rows); the result is read from lane 63.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+  const uint32_t id = 0xFFFFFFFFu;
+  uint32_t t;
+  t = __builtin_amdgcn_update_dpp(id, v, 0x111, 0xF, 0xF, false); v = v < t ? v : t;
+  t = __builtin_amdgcn_update_dpp(id, v, 0x112, 0xF, 0xF, false); v = v < t ? v : t;
+  t = __builtin_amdgcn_update_dpp(id, v, 0x114, 0xF, 0xF, false); v = v < t ? v : t;
+  t = __builtin_amdgcn_update_dpp(id, v, 0x118, 0xF, 0xF, false); v = v < t ? v : t;
+  t = __builtin_amdgcn_update_dpp(id, v, 0x142, 0xA, 0xF, false); v = v < t ? v : t;
+  t = __builtin_amdgcn_update_dpp(id, v, 0x143, 0xC, 0xF, false); v = v < t ? v : t;
+  return __builtin_amdgcn_readlane(v, 63);
 }
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    uint32_t o = __shfl_xor(v, off, 64);
-    v = o > v ? o : v;
-  }
-  return v;
+  uint32_t t;
+  t = __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, false); v = v > t ? v : t;
+  t = __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, false); v = v > t ? v : t;
+  t = __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, false); v = v > t ? v : t;
+  t = __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, false); v = v > t ? v : t;
+  t = __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, false); v = v > t ? v : t;
+  t = __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xC, 0xF, false); v = v > t ? v : t;
+  return __builtin_amdgcn_readlane(v, 63);
+}
+// Population count of a set whose words are held one per lane (lanes < W).
+__device__ __forceinline__ uint32_t set_popc(uint64_t wk, uint32_t W) {
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < W; ++k) s += (uint32_t)__popcll(readlane64(wk, k));
+  return s;
 }
 // Number of set bits of `mask` below this lane.
 __device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
@@ -115,6 +123,63 @@ __device__ __forceinline__ void for_each_p_in_nu(const KArgs &a, uint32_t u, con
 // Per-wave accumulators (lane 0 flushes them at exit).  `words` counts the
 // 64-bit bitmap words the search reads; `build` the bytes the subproblem
 // build moves (both adjacency lists, the candidate list, the bitmap).
+// Calls on_hit(j) for every j in [j0, p), ascending, with P[j] in the sorted
+// run col[b, e); stops early when on_hit returns true.  Binary search of the
+// shorter side into the longer when their lengths differ by 8x or more, a
+// linear merge otherwise.
+template <class F>
+__device__ __forceinline__ void match_run(const uint32_t *col, uint64_t b, uint64_t e, const uint32_t *P, uint32_t j0,
+                                          uint32_t p, F on_hit) {
+  if (b >= e || j0 >= p) return;
+  const uint64_t n = e - b;
+  const uint32_t m = p - j0;
+  if ((uint64_t)m * 8 < n) {
+    uint64_t pos = b;
+    for (uint32_t j = j0; j < p && pos < e; ++j) {
+      const uint32_t key = P[j];
+      pos = lower_bound(col, pos, e, key);
+      if (pos < e && col[pos] == key && on_hit(j)) return;
+    }
+  } else if (n * 8 < (uint64_t)m) {
+    uint32_t lo = j0;
+    for (uint64_t i = b; i < e && lo < p; ++i) {
+      const uint32_t w = col[i];
+      uint32_t x = lo, z = p;
+      while (x < z) {
+        uint32_t mid = (x + z) >> 1;
+        if (P[mid] < w) x = mid + 1;
+        else z = mid;
+      }
+      lo = x;
+      if (x < p && P[x] == w) {
+        if (on_hit(x)) return;
+        lo = x + 1;
+      }
+    }
+  } else {
+    uint64_t i = b;
+    uint32_t j = j0;
+    while (i < e && j < p) {
+      const uint32_t x = col[i], y = P[j];
+      if (x < y) ++i;
+      else if (x > y) ++j;
+      else {
+        if (on_hit(j)) return;
+        ++i;
+        ++j;
+      }
+    }
+  }
+}
+
+// Relaxed 64-bit OR into the bitmap (HBM or LDS).
+__device__ __forceinline__ void or64(uint64_t *p, uint64_t v) {
+  __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void or64(__attribute__((address_space(3))) uint64_t *p, uint64_t v) {
+  __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 struct Acc {
   unsigned long long count;
   unsigned long long hash;
@@ -251,7 +316,7 @@ __device__ Lvl make_child(const Sub &s, const Lvl &par, XxP par_xx, uint32_t c, 
   ch.key = par.key + gP[c];
   ch.rsize = par.rsize + 1;
   uint64_t mk = 0;
-  uint32_t psz = wave_sum_u32((uint32_t)__popcll(ch.pk));
+  uint32_t psz = set_popc(ch.pk, W);
   if (psz > 1) {
     for (uint32_t j = 0; j < W; ++j) {
       uint64_t w = readlane64(ch.pk, j);
@@ -374,7 +439,8 @@ __device__ __forceinline__ void publish_bit(const KArgs &a, uint32_t w, bool set
 // Returns the number of nodes this wave evaluated.
 template <class BmP, class GpP, class StP>
 __device__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm, GpP gP, StP stk, uint64_t *stk_g, Acc &acc,
-                            unsigned long long *s_hist, uint32_t me, bool &published, bool eager = false) {
+                            unsigned long long *s_hist, uint32_t me, bool &published, bool eager = false,
+                            bool hbm_pending = false) {
   constexpr bool kStkLds = !std::is_same<StP, uint64_t *>::value;
   const uint32_t W = s.W, p = s.p, rp = s.rows_pad, LS = s.LS;
   const uint32_t l = lane();
@@ -397,8 +463,15 @@ __device__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm, GpP gP, StP st
       if (bcast32(go)) {
         const uint32_t lp = (uint32_t)(S + 1);
         StP src = stk + (uint64_t)lp * LS;
-        uint32_t n = wave_sum_u32(wl ? (uint32_t)__popcll(src[3 * W + l]) : 0u);
+        uint32_t n = set_popc(wl ? (uint64_t)src[3 * W + l] : 0ull, W);
         if (n > 0) {
+          if (hbm_pending) {  // bitmap and label hashes exist only in LDS so far
+            uint64_t *dbm = (uint64_t *)s.bm, *dgp = (uint64_t *)s.gP;
+            const uint64_t nb = (uint64_t)W * rp;
+            for (uint64_t i = l; i < nb; i += 64) dbm[i] = bm[i];
+            for (uint64_t i = l; i < p; i += 64) dgp[i] = gP[i];
+            hbm_pending = false;
+          }
           const uint64_t *snap_g;
           if (kStkLds) {
             uint64_t *dst = stk_g + (uint64_t)lp * LS;
@@ -424,14 +497,15 @@ __device__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm, GpP gP, StP st
     }
     if (st == 0) {
       ++nodes;
-      uint32_t psz = wave_sum_u32((uint32_t)__popcll(L.pk));
+      uint32_t psz = set_popc(L.pk, W);
       if (psz == 0) {
         if (__ballot(L.xpk != 0) == 0 && L.xxcnt == 0) emit(a, s, acc, s_hist, s.base_size + L.rsize, L.key, L.rk);
         st = 2;
         continue;
       }
-      // Pivot: argmin over candidate rows u of (|P \ N(u)|, u).
-      uint64_t best = ~0ull;
+      // Pivot: argmin over candidate rows u of (|P \ N(u)|, u).  Each lane
+      // visits its rows in increasing u, so a strict < keeps its lowest u.
+      uint32_t best_c = ~0u, best_u = ~0u;
       for (uint32_t j = 0; j < W; ++j) {
         uint64_t cw = readlane64(L.pk | L.xpk, j);
         if (cw == 0) continue;
@@ -440,8 +514,10 @@ __device__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm, GpP gP, StP st
           uint32_t u = j * 64 + l;
           uint32_t cnt = 0;
           for (uint32_t k = 0; k < W; ++k) cnt += __popcll(readlane64(L.pk, k) & ~bm[(uint64_t)k * rp + u]);
-          uint64_t kk = ((uint64_t)cnt << 32) | u;
-          best = kk < best ? kk : best;
+          if (cnt < best_c) {
+            best_c = cnt;
+            best_u = u;
+          }
         }
       }
       StP XX = rec + 4 * W;
@@ -453,16 +529,18 @@ __device__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm, GpP gP, StP st
           uint32_t u = p + j * 64 + l;
           uint32_t cnt = 0;
           for (uint32_t k = 0; k < W; ++k) cnt += __popcll(readlane64(L.pk, k) & ~bm[(uint64_t)k * rp + u]);
-          uint64_t kk = ((uint64_t)cnt << 32) | u;
-          best = kk < best ? kk : best;
+          if (cnt < best_c) {
+            best_c = cnt;
+            best_u = u;
+          }
         }
       }
-      best = wave_min_u64(best);
-      if ((best >> 32) == 0) {  // an excluded row covers P: no maximal clique below
+      const uint32_t min_c = wave_min_u32(best_c);
+      if (min_c == 0) {  // an excluded row covers P: no maximal clique below
         st = 2;
         continue;
       }
-      const uint32_t piv = (uint32_t)best;
+      const uint32_t piv = wave_min_u32(best_c == min_c ? best_u : ~0u);
       L.vk = wl ? (L.pk & ~bm[(uint64_t)l * rp + piv]) : 0;
       acc.words += W;
       if (wl) rec[3 * W + l] = L.vk;
@@ -576,67 +654,107 @@ __device__ __forceinline__ void flush_graph(const KArgs &a, uint32_t g, Acc &acc
   __builtin_amdgcn_wave_barrier();
 }
 
-// Builds root `canon` in this wave's data region and searches it.
-// Returns tree nodes evaluated by this wave.
+// Builds root `canon` and searches it.  Returns tree nodes evaluated by this
+// wave.
+//
+// In degeneracy order every vertex has at most `degeneracy` higher-ordered
+// neighbours, so the build touches only short out-lists N+(.) (their start
+// is outoff[v]) wherever it can:
+//   P  = N+(v1) & N+(v0) (edge roots; all of P lies above v1 > v0),
+//   X  = N(v0) & N-(v1) below v1 (only its emptiness when P is empty),
+//   X' = rows x of X with N+(x) & P non-empty (x < v1 < every P vertex),
+//   bitmap rows: P x P from N+(u) for u in P (set symmetrically), X' rows
+//   from N+(x) above v1.
+// The bitmap is built in this wave's LDS when it fits, else in its HBM data
+// region; an LDS-built bitmap is copied to HBM only when stealing needs it.
 __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk_g, lds64 *lds, uint32_t me,
                                uint64_t canon, Acc &acc, unsigned long long *s_hist, bool &published,
                                bool eager) {
   const uint32_t l = lane();
   const uint64_t desc = a.roots[canon];
-  uint32_t *C = (uint32_t *)slot;
+  const uint32_t *col = a.col;
+  uint32_t *Pl = (uint32_t *)slot;  // P list, then X (later X') list
   const uint64_t ccap = a.slot_bytes / 4;
   Sub s;
   s.canon = canon;
   s.bm_owner = me;
   s.graph = a.root_graph ? a.root_graph[canon] : 0;
-  uint64_t c = 0, x = 0;
+  uint64_t p = 0, x = 0;
+  uint32_t above;  // every P vertex lies above this vertex, every X vertex below it
   if (a.root_mode == 2) {
     s.v0 = (uint32_t)(desc >> 32);
     s.v1 = (uint32_t)desc;
     s.base_size = 2;
     s.base_key = a.label_hash[s.v0] + a.label_hash[s.v1];
-    // C = N(v0) ^ N(v1): lanes take 64 entries of the shorter list at a time
-    // and binary-search them in the longer one (BkPivotBitBalance.cuh:449-460).
-    const uint32_t *A = a.col + a.rowoff[s.v0], *B = a.col + a.rowoff[s.v1];
-    uint64_t na = a.rowoff[s.v0 + 1] - a.rowoff[s.v0], nb = a.rowoff[s.v1 + 1] - a.rowoff[s.v1];
-    if (na > nb) {
-      const uint32_t *t = A;
-      A = B;
-      B = t;
-      uint64_t tn = na;
-      na = nb;
-      nb = tn;
-    }
-    for (uint64_t base = 0; base < na; base += 64) {
-      uint64_t i = base + l;
-      uint32_t val = 0;
-      bool found = false;
-      if (i < na) {
-        val = A[i];
-        uint64_t pos = lower_bound(B, 0, nb, val);
-        found = pos < nb && B[pos] == val;
+    above = s.v1;
+    const uint64_t b0 = a.rowoff[s.v0], e0 = a.rowoff[s.v0 + 1], o0 = a.outoff[s.v0];
+    const uint64_t b1 = a.rowoff[s.v1], e1 = a.rowoff[s.v1 + 1], o1 = a.outoff[s.v1];
+    // P = N+(v1) & (N+(v0) above v1): iterate the shorter run, binary-search
+    // the longer (BkPivotBitBalance.cuh:449-469 builds it from N(v0) & N(v1)).
+    {
+      const uint64_t s0 = lower_bound(col, o0, e0, s.v1 + 1);
+      uint64_t ab = o1, ae = e1, bb = s0, be = e0;
+      if (ae - ab > be - bb) {
+        uint64_t t0 = ab, t1 = ae;
+        ab = bb;
+        ae = be;
+        bb = t0;
+        be = t1;
       }
-      uint64_t m = __ballot(found);
-      if (found) {
-        uint64_t at = c + rank_below(m);
-        if (at < ccap) C[at] = val;
+      for (uint64_t base = ab; base < ae; base += 64) {
+        const uint64_t i = base + l;
+        uint32_t val = 0;
+        bool found = false;
+        if (i < ae) {
+          val = col[i];
+          const uint64_t pos = lower_bound(col, bb, be, val);
+          found = pos < be && col[pos] == val;
+        }
+        const uint64_t m = __ballot(found);
+        if (found && p + rank_below(m) < ccap) Pl[p + rank_below(m)] = val;
+        p += __popcll(m);
       }
-      c += __popcll(m);
-      x += __popcll(__ballot(found && val < s.v1));
+      acc.build += 4 * ((e1 - o1) + (e0 - s0));
     }
-    acc.build += 4 * (na + nb);
+    // X = (N(v0) below v1) & N-(v1); with P empty only its emptiness matters.
+    {
+      uint64_t ab = b0, ae = lower_bound(col, b0, e0, s.v1), bb = b1, be = o1;
+      if (ae - ab > be - bb) {
+        uint64_t t0 = ab, t1 = ae;
+        ab = bb;
+        ae = be;
+        bb = t0;
+        be = t1;
+      }
+      acc.build += 4 * ((ae - ab) + (be - bb));
+      for (uint64_t base = ab; base < ae; base += 64) {
+        const uint64_t i = base + l;
+        uint32_t val = 0;
+        bool found = false;
+        if (i < ae) {
+          val = col[i];
+          const uint64_t pos = lower_bound(col, bb, be, val);
+          found = pos < be && col[pos] == val;
+        }
+        const uint64_t m = __ballot(found);
+        if (found && p + x + rank_below(m) < ccap) Pl[p + x + rank_below(m)] = val;
+        x += __popcll(m);
+        if (p == 0 && x) break;
+      }
+    }
   } else {
     s.v0 = (uint32_t)desc;
     s.v1 = 0;
     s.base_size = 1;
     s.base_key = a.label_hash[s.v0];
-    const uint32_t *N = a.col + a.rowoff[s.v0];
-    c = a.rowoff[s.v0 + 1] - a.rowoff[s.v0];
-    x = lower_bound(N, 0, c, s.v0);
-    for (uint64_t i = l; i < c && i < ccap; i += 64) C[i] = N[i];
-    acc.build += 4 * c;
+    above = s.v0;
+    const uint64_t b = a.rowoff[s.v0], e = a.rowoff[s.v0 + 1], o = a.outoff[s.v0];
+    p = e - o;
+    x = o - b;
+    for (uint64_t i = l; i < p && i < ccap; i += 64) Pl[i] = col[o + i];
+    for (uint64_t i = l; i < x && p + i < ccap; i += 64) Pl[p + i] = col[b + i];
+    acc.build += 4 * (e - b);
   }
-  const uint64_t p = c - x;
   if (p == 0) {
     if (x == 0) {
       s.W = 0;
@@ -650,36 +768,39 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk_g, l
     return 0;
   }
   const uint64_t W = (p + 63) / 64;
-  if (c > ccap) {  // candidate list alone exceeds the data region
+  if (p + x > ccap) {  // candidate lists alone exceed the data region
+    const uint64_t c = p + x;
     defer_root(a, canon, align_up(4 * c, 256) + align_up(8 * W * align_up(c, 64), 256) + align_up(8 * p, 256),
                (p + 1) * (4 * W + (x + 63) / 64 + 4) * 8);
     return 0;
   }
   wave_fence();
-  const uint32_t *Plist = C + x;
-  // X0' = X0 rows with at least one neighbour in P0, compacted in place
-  // (graph_gpu.cuh:90-106).
+  const uint32_t *P = Pl;
+  uint32_t *X = Pl + p;
+  // X' = X rows with a neighbour in P, compacted in place (graph_gpu.cuh:90-106).
   uint64_t xq = 0;
   for (uint64_t base = 0; base < x; base += 64) {
-    uint64_t i = base + l;
+    const uint64_t i = base + l;
     bool keep = false;
     uint32_t xv = 0;
     if (i < x) {
-      xv = C[i];
-      for_each_p_in_nu(a, xv, Plist, (uint32_t)p, [&](uint32_t) {
+      xv = X[i];
+      const uint64_t e = a.rowoff[xv + 1];
+      const uint64_t rb = lower_bound(col, a.outoff[xv], e, above + 1);
+      match_run(col, rb, e, P, 0, (uint32_t)p, [&](uint32_t) {
         keep = true;
         return true;
       });
     }
-    uint64_t m = __ballot(keep);
-    if (keep) C[xq + rank_below(m)] = xv;
+    const uint64_t m = __ballot(keep);
+    if (keep) X[xq + rank_below(m)] = xv;
     xq += __popcll(m);
   }
   wave_fence();
   const uint32_t Wx = (uint32_t)((xq + 63) / 64);
   const uint64_t rows = p + xq;
   const uint32_t rows_pad = (uint32_t)align_up(rows, 64);
-  const uint64_t bm_off = align_up(4 * c, 256);
+  const uint64_t bm_off = align_up(4 * (p + x), 256);
   const uint64_t gp_off = bm_off + align_up(8ull * W * rows_pad, 256);
   const uint64_t data_need = gp_off + align_up(8 * p, 256);
   const uint32_t LS = (uint32_t)(4 * W + Wx + 4);
@@ -687,34 +808,56 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk_g, l
     defer_root(a, canon, data_need, (p + 1) * LS * 8);
     return 0;
   }
-  uint64_t *bm = (uint64_t *)(slot + bm_off);
-  uint64_t *gP = (uint64_t *)(slot + gp_off);
-  acc.build += 4 * c + 8 * W * rows_pad;
-  // Bitmap (word-major): row i is P0[i] for i < p, else X0'[i - p].
-  for (uint64_t i = l; i < W * rows_pad; i += 64) bm[i] = 0;
-  for (uint64_t i = l; i < p; i += 64) gP[i] = a.label_hash[Plist[i]];
-  wave_fence();
+  uint64_t *bm_g = (uint64_t *)(slot + bm_off);
+  uint64_t *gp_g = (uint64_t *)(slot + gp_off);
+  const uint64_t nbw = W * rows_pad;
+  const bool in_lds = a.lds_tier && (nbw + p) * 8 <= a.lds_bytes;
+  acc.build += 4 * (p + x) + 8 * nbw;
+  // Bitmap (word-major): row i is P[i] for i < p, else X'[i - p].
   uint32_t maxdeg = 0;
-  for (uint64_t i = l; i < rows; i += 64) {
-    uint32_t u = i < p ? Plist[i] : C[i - p];
-    uint32_t curw = ~0u, deg = 0;
-    uint64_t accw = 0;
-    for_each_p_in_nu(a, u, Plist, (uint32_t)p, [&](uint32_t b) {
-      uint32_t w = b >> 6;
-      if (w != curw) {
-        if (curw != ~0u) bm[(uint64_t)curw * rows_pad + i] = accw;
-        curw = w;
-        accw = 0;
+  // One build path through flat pointers (LDS or HBM resolved at run time):
+  // a second, LDS-typed instantiation would double this code in the kernel
+  // and its register allocation.
+  auto build = [&](uint64_t *T, uint64_t *G) {
+    for (uint64_t i = l; i < nbw; i += 64) T[i] = 0;
+    for (uint64_t i = l; i < p; i += 64) G[i] = a.label_hash[P[i]];
+    wave_fence();
+    for (uint64_t i = l; i < rows; i += 64) {
+      if (i < p) {
+        const uint32_t u = P[i];
+        match_run(col, a.outoff[u], a.rowoff[u + 1], P, (uint32_t)i + 1, (uint32_t)p, [&](uint32_t j) {
+          or64(&T[(uint64_t)(j >> 6) * rows_pad + i], 1ull << (j & 63));
+          or64(&T[(uint64_t)(i >> 6) * rows_pad + j], 1ull << (i & 63));
+          return false;
+        });
+      } else {
+        const uint32_t u = X[i - p];
+        const uint64_t e = a.rowoff[u + 1];
+        const uint64_t rb = lower_bound(col, a.outoff[u], e, above + 1);
+        uint32_t curw = ~0u;
+        uint64_t accw = 0;
+        match_run(col, rb, e, P, 0, (uint32_t)p, [&](uint32_t b) {
+          const uint32_t w = b >> 6;
+          if (w != curw) {
+            if (curw != ~0u) T[(uint64_t)curw * rows_pad + i] = accw;
+            curw = w;
+            accw = 0;
+          }
+          accw |= 1ull << (b & 63);
+          return false;
+        });
+        if (curw != ~0u) T[(uint64_t)curw * rows_pad + i] = accw;
       }
-      accw |= 1ull << (b & 63);
-      ++deg;
-      return false;
-    });
-    if (curw != ~0u) bm[(uint64_t)curw * rows_pad + i] = accw;
-    if (i < p) maxdeg = deg > maxdeg ? deg : maxdeg;
-  }
-  maxdeg = wave_max_u32(maxdeg);
-  wave_fence();
+    }
+    wave_fence();
+    for (uint64_t i = l; i < p; i += 64) {
+      uint32_t d = 0;
+      for (uint64_t w = 0; w < W; ++w) d += (uint32_t)__popcll(T[w * rows_pad + i]);
+      maxdeg = d > maxdeg ? d : maxdeg;
+    }
+    maxdeg = wave_max_u32(maxdeg);
+  };
+  build(in_lds ? (uint64_t *)lds : bm_g, in_lds ? (uint64_t *)(lds + nbw) : gp_g);
   // Depth bound: a clique inside P0 has at most maxdeg + 1 vertices.
   const uint64_t levels = (p < (uint64_t)maxdeg + 1 ? p : (uint64_t)maxdeg + 1) + 1;
   if (levels * LS * 8 > a.stack_bytes) {
@@ -728,9 +871,9 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk_g, l
   s.rows_pad = rows_pad;
   s.LS = LS;
   s.levels = (uint32_t)levels;
-  s.Plist = Plist;
-  s.bm = bm;
-  s.gP = gP;
+  s.Plist = P;
+  s.bm = bm_g;
+  s.gP = gp_g;
   // Level 0: R = {}, P = P0, XP = {}, XX = X0'.
   Lvl L0;
   L0.rk = 0;
@@ -742,18 +885,22 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk_g, l
   L0.xxcnt = (uint32_t)xq;
   L0.xlo = 0;
   L0.xhi = Wx;
-  if (a.lds_tier && lds_words(s) * 8 <= a.lds_bytes) {
-    lds64 *st = stage_lds(s, lds);
+  if (in_lds && lds_words(s) * 8 <= a.lds_bytes) {
+    lds64 *st = lds + nbw + p;
     for (uint32_t j = l; j < Wx; j += 64) st[4 * W + j] = low_mask(xq - (uint64_t)j * 64);
     store_rpx(st, s, L0);
     wave_fence();
-    return run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + W * rows_pad), st, stk_g, acc, s_hist, me,
-                   published, eager);
+    return run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + nbw), st, stk_g, acc, s_hist, me, published,
+                   eager, true);
+  }
+  if (in_lds) {  // the stack does not fit next to it: search from HBM
+    for (uint64_t i = l; i < nbw; i += 64) bm_g[i] = lds[i];
+    for (uint64_t i = l; i < p; i += 64) gp_g[i] = lds[nbw + i];
   }
   for (uint32_t j = l; j < Wx; j += 64) stk_g[4 * W + j] = low_mask(xq - (uint64_t)j * 64);
   store_rpx(stk_g, s, L0);
   wave_fence();
-  return run_dfs(a, s, (const uint64_t *)bm, (const uint64_t *)gP, stk_g, stk_g, acc, s_hist, me, published,
+  return run_dfs(a, s, (const uint64_t *)bm_g, (const uint64_t *)gp_g, stk_g, stk_g, acc, s_hist, me, published,
                  eager);
 }
 
