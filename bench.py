@@ -35,31 +35,10 @@ METRIC = "MCE wall time & maximal cliques/s, 1 MI355X; batched+learned vs plain"
 
 
 def build_workload(name: str):
-    import numpy as np
-    import rdmce_amd as R
+    from rdmce_amd import workloads
 
-    if name.startswith("rmat20-core"):
-        core_max = int(name[len("rmat20-core"):])
-        g = R.Graph.rmat(20, 16, 0.57, 0.19, 0.19, seed=2)
-        g.SortByOrder(R.OrderType.DEG)
-        core = g.GetCoreNumbers()
-        roots = g.roots(R.ROOT_EDGE)
-        v0 = (roots >> np.uint64(32)).astype(np.int64)
-        end = int(np.searchsorted(core[v0], core_max, side="right"))
-        # plain schedule: canonical roots in reverse order (high core first),
-        # as the reference dispatches edge slots (BkPivotBitBalance.cuh:429-431)
-        ids = np.arange(end, dtype=np.uint64)[::-1].copy()
-        desc = {"workload": name, "graph": "R-MAT scale 20, ef 16, (0.57,0.19,0.19,0.05), seed 2, degeneracy order",
-                "n": g.GetNumVertices(), "m": g.GetNumEdges(), "degeneracy": g.GetDegeneracy(),
-                "roots": end, "root_rule": f"edge roots (v0<v1) with core(v0) <= {core_max}"}
-        return g, ids, desc
-    if name == "er2048":
-        g = R.Graph.erdos_renyi(2048, 0.008, 1)
-        g.SortByOrder(R.OrderType.DEG)
-        m = g.GetNumEdges()
-        ids = np.arange(m, dtype=np.uint64)[::-1].copy()
-        return g, ids, {"workload": name, "graph": "G(2048, 0.008) seed 1", "roots": m}
-    raise SystemExit(f"unknown workload {name}")
+    w = workloads.load(name)
+    return w.graph, w.plain_order(), w.desc
 
 
 def read_traffic(workload: str):

@@ -109,7 +109,8 @@ mce_status mce_gen_batch_params(uint64_t seed, uint64_t index, uint32_t nmin, ui
 typedef struct mce_options {
   int32_t device;            /* HIP device ordinal (MceGpuSolver(g, device_id), mce_gpu.cu:9-12) */
   int32_t root_mode;         /* MCE_ROOT_EDGE (default) or MCE_ROOT_VERTEX */
-  int32_t waves_per_cu;      /* resident wavefronts per CU; 0 = default (WARP_PER_SM analogue) */
+  int32_t waves_per_cu;      /* resident wavefronts per CU (1..32; 0 = 16), WARP_PER_SM analogue; also picks the
+                                kernel compiled for that occupancy (<=16, <=20, <=24, <=32) */
   int32_t share_bound;       /* TASK_SHARE_BOUND: |P| at which a level is exposed; 0 = 24; <0 = off */
   int32_t want_list;         /* 1 = return the clique list */
   int32_t want_root_stats;   /* 1 = return per-root node counts and wave cycles */

@@ -12,7 +12,8 @@ from ctypes import (POINTER, Structure, c_char_p, c_double, c_int32, c_uint32,
                     c_uint64, c_void_p)
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "librdmce.so")
+# RDMCE_LIB names another build of the library (e.g. to compare two builds).
+LIB_PATH = os.environ.get("RDMCE_LIB") or os.path.join(PKG_DIR, "librdmce.so")
 
 MCE_OK = 0
 MCE_ERR_ARG = 1

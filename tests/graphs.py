@@ -111,18 +111,6 @@ def hist_of(cliques):
 
 
 def expected_er_cliques(n, p):
-    """Closed-form expectation of maximal cliques of size >= 2 in G(n, p):
-    sum_k C(n,k) p^C(k,2) (1 - p^k)^(n-k), summed in log space."""
-    if p <= 0.0:
-        return 0.0
-    if p >= 1.0:
-        return 1.0
-    tot = 0.0
-    for k in range(2, n + 1):
-        pk = p ** k
-        if pk >= 1.0:
-            continue
-        lt = (math.lgamma(n + 1) - math.lgamma(k + 1) - math.lgamma(n - k + 1) + math.comb(k, 2) * math.log(p)
-              + (n - k) * math.log1p(-pk))
-        tot += math.exp(lt) if lt > -700 else 0.0
-    return tot
+    """Closed-form expectation of maximal cliques of size >= 2 in G(n, p)."""
+    from rdmce_amd.workloads import expected_er_cliques as e
+    return e(n, p)

@@ -171,7 +171,7 @@ def test_sharing_single_heavy_root():
 
 def test_rmat20_core499_full_size_golden():
     """The bench workload at full size against the committed CPU-oracle
-    golden (tests/golden/make_rmat_golden.py): 13.2M roots, ~1.0e9 cliques."""
+    golden (tests/golden/make_workload_golden.py): 13.2M roots, ~1.0e9 cliques."""
     import json
     banner("rmat20 core<=499 full size")
     with open(os.path.join(GOLDEN, "rmat20_s2_core499.json")) as f:
@@ -184,6 +184,24 @@ def test_rmat20_core499_full_size_golden():
     end = int(np.searchsorted(core[v0], gd["core_max"], side="right"))
     assert end == gd["roots"]
     out = R.solve(g, root_order=np.arange(end, dtype=np.uint64)[::-1].copy())
+    assert out.count == gd["count"]
+    assert out.hash == gd["hash"]
+    assert out.hist_dict() == {int(k): v for k, v in gd["hist"].items()}
+    assert out.total_nodes == gd["tree_nodes"]
+
+
+def test_cl4m_core300_full_size_golden():
+    """Chung-Lu power-law graph with planted communities (n = 4e6), every
+    edge root with core(v0) <= 300 (48.2M roots, 2.19e9 cliques), against
+    the committed CPU-oracle golden (tests/golden/make_workload_golden.py)."""
+    import json
+    from rdmce_amd import workloads
+    banner("cl4m core<=300 full size")
+    with open(os.path.join(GOLDEN, "cl4m_s4_core300.json")) as f:
+        gd = json.load(f)
+    w = workloads.load("cl4m-core300")
+    assert w.n_roots == gd["roots"]
+    out = R.solve(w.graph, root_order=w.plain_order())
     assert out.count == gd["count"]
     assert out.hash == gd["hash"]
     assert out.hist_dict() == {int(k): v for k, v in gd["hist"].items()}
