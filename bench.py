@@ -68,14 +68,44 @@ def cpu_baseline(g, ids, stride: int, threads: int):
                       f"in {dt:.2f} s on {threads} host threads"}
 
 
+def time_plan(plan, steps: int, warmup: int, label: str):
+    """Runs `warmup` untimed and `steps` timed passes of a resident plan;
+    returns (outputs, wall seconds of the timed passes)."""
+    import numpy as np
+
+    from rdmce_amd._lib import check, lib
+
+    for i in range(warmup):
+        o = plan.run()
+        print(f"[bench] {label} warmup {i}: {o.count} cliques, kernel {o.kernel_ms:.1f} ms", file=sys.stderr,
+              flush=True)
+    check(lib().mce_device_sync(0))
+    t0 = time.perf_counter()
+    outs = []
+    for i in range(steps):
+        outs.append(plan.run())
+        print(f"[bench] {label} step {i}: kernel {outs[-1].kernel_ms:.1f} ms", file=sys.stderr, flush=True)
+    check(lib().mce_device_sync(0))
+    wall = time.perf_counter() - t0
+    first = outs[0]
+    for o in outs[1:]:
+        if (o.count, o.hash) != (first.count, first.hash) or not np.array_equal(o.hist, first.hist):
+            raise SystemExit(f"{label} steps disagree: results are not reproducible")
+    return outs, wall
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rmat20-core499")
+    ap.add_argument("--mode", choices=["tuned", "plain"], default="tuned",
+                    help="tuned: time the plain path, then the learned schedule with autotuned settings "
+                         "(value); plain: the plain path only")
     ap.add_argument("--share-bound", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--tune-stride", type=int, default=1, help="autotune on every n-th root (1: all)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=16)
     ap.add_argument("--copy-probe", action="store_true", help="also launch the traffic calibration copy kernel")
@@ -89,32 +119,46 @@ def main():
 
     import numpy as np
     import rdmce_amd as R
+    from rdmce_amd import learned
     from rdmce_amd._lib import check, lib
 
     t_prep = time.perf_counter()
     g, ids, desc = build_workload(args.workload)
     prep_s = time.perf_counter() - t_prep
+    n_roots = int(ids.size)
     print(f"[bench] workload {desc} prepared in {prep_s:.1f}s", file=sys.stderr, flush=True)
 
+    # plain path: reverse canonical order, default settings (the reference's dispatch)
     plan = R.Plan(g, root_order=ids, share_bound=args.share_bound, waves_per_cu=args.waves_per_cu)
-    for i in range(args.warmup):
-        o = plan.run()
-        print(f"[bench] warmup {i}: {o.count} cliques, kernel {o.kernel_ms:.1f} ms", file=sys.stderr, flush=True)
-    check(lib().mce_device_sync(0))
-    t0 = time.perf_counter()
-    outs = []
-    for i in range(args.steps):
-        outs.append(plan.run())
-        print(f"[bench] step {i}: kernel {outs[-1].kernel_ms:.1f} ms", file=sys.stderr, flush=True)
-    check(lib().mce_device_sync(0))
-    t1 = time.perf_counter()
+    plain_outs, plain_wall = time_plan(plan, args.steps, args.warmup, "plain")
     plan.free()
-    first = outs[0]
-    for o in outs[1:]:
-        if (o.count, o.hash) != (first.count, first.hash) or not np.array_equal(o.hist, first.hist):
-            raise SystemExit("steps disagree: results are not reproducible")
+    plain = plain_outs[0]
+    plain_info = {"value": plain.count * args.steps / plain_wall, "ms_per_step": plain_wall * 1e3 / args.steps,
+                  "kernel_ms": sum(o.kernel_ms for o in plain_outs) / len(plain_outs),
+                  "schedule": "reverse canonical order (high core first)", "waves_per_cu": 16,
+                  "share_bound": args.share_bound or 24}
 
-    wall = t1 - t0
+    outs, wall, sched_info = plain_outs, plain_wall, {"schedule": "plain"}
+    if args.mode == "tuned":
+        t = time.perf_counter()
+        pred, model, rep = learned.fit_and_predict(g, n_roots)
+        sched = learned.learned_schedule(pred, np.arange(n_roots, dtype=np.uint64))
+        plan = R.Plan(g, root_order=sched.order, eager=sched.eager)
+        tune = plan.autotune(stride=args.tune_stride)
+        prep2 = time.perf_counter() - t
+        print(f"[bench] learned+tuned prep {prep2:.1f}s: fit {rep}, tune {tune}", file=sys.stderr, flush=True)
+        outs, wall = time_plan(plan, args.steps, args.warmup, "tuned")
+        plan.free()
+        o = outs[0]
+        if (o.count, o.hash) != (plain.count, plain.hash) or not np.array_equal(o.hist, plain.hist):
+            raise SystemExit("learned+tuned results differ from the plain path")
+        sched_info = {"schedule": "learned (longest predicted first)" if sched.learned else "plain (fallback)",
+                      "learned": bool(sched.learned), "model_loss": rep.loss, "label_roots": rep.sample_roots,
+                      "waves_per_cu": tune["waves_per_cu"], "share_bound": tune["share_bound"],
+                      "lds_tier": tune["lds_tier"], "tune_trials": tune["trials"],
+                      "tune_stride": args.tune_stride, "learn_tune_s": round(prep2, 2)}
+
+    first = outs[0]
     value = first.count * args.steps / wall
     kernel_ms = sum(o.kernel_ms for o in outs) / len(outs)
     algo_bytes = first.search_bytes + first.build_bytes
@@ -140,7 +184,9 @@ def main():
         "data": "synthetic",
         "config": dict(desc, cliques=first.count, clique_hash=first.hash, tree_nodes=first.total_nodes,
                        kernel_ms=kernel_ms, shared_tasks=first.shared_tasks, passes=first.passes,
-                       waves=first.waves, schedule="plain (reverse root order)", prep_s=round(prep_s, 2)),
+                       waves=first.waves, prep_s=round(prep_s, 2), mode=args.mode, **sched_info),
+        "plain": plain_info,
+        "speedup_vs_plain": value / plain_info["value"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(args.workload),
                      "algo_bytes": algo_bytes},

@@ -579,7 +579,7 @@ mce_status plan_create_impl(const PlanSpec &spec, const mce_options &opt, mce_pl
   }
 
   const int wpc = opt.waves_per_cu > 0 ? opt.waves_per_cu : 16;
-  p->max_waves = (uint32_t)std::min<uint64_t>(16384, (uint64_t)p->cus * std::max(wpc, 24));
+  p->max_waves = (uint32_t)std::min<uint64_t>(16384, (uint64_t)p->cus * std::max(wpc, 32));
   size_t free_b = 0, total_b = 0;
   PTRY(hipMemGetInfo(&free_b, &total_b));
   // Each wave gets a data region and a stack; both are sized for the largest
@@ -684,7 +684,7 @@ hipError_t enqueue_run(const mce_plan *p, const KArgs &a, hipStream_t s) {
   if (e == hipSuccess && p->d_gcount) e = launch_fill_zero(p->d_gcount, p->n_graphs, s);
   if (e == hipSuccess && p->d_ghash) e = launch_fill_zero(p->d_ghash, p->n_graphs, s);
   if (e == hipSuccess && p->d_ghist) e = launch_fill_zero(p->d_ghist, p->n_graphs * p->g_hist_stride, s);
-  if (e == hipSuccess) e = launch_mce(a, p->waves, s);
+  if (e == hipSuccess) e = launch_mce(a, p->waves, (uint32_t)p->waves_per_cu, s);
   return e;
 }
 
@@ -785,7 +785,7 @@ mce_status run_core(mce_plan *p, const KArgs &a0, bool allow_graph, RunOut &ro) 
     b.stack_bytes = stack2;
     b.n_waves = (uint32_t)waves2;
     HIP_TRY(hipEventRecord(p->ev[2], s));
-    HIP_TRY(launch_mce(b, (uint32_t)waves2, s));
+    HIP_TRY(launch_mce(b, (uint32_t)waves2, (uint32_t)p->waves_per_cu, s));
     HIP_TRY(hipEventRecord(p->ev[3], s));
     HIP_TRY(hipEventSynchronize(p->ev[3]));
     HIP_TRY(hipEventElapsedTime(&ms, p->ev[2], p->ev[3]));
@@ -869,8 +869,10 @@ mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
   uint64_t *d_sample = nullptr;
   HIP_TRY(dalloc(&d_sample, sample.size()));
   if (!sample.empty()) HIP_TRY(hipMemcpy(d_sample, sample.data(), sample.size() * 8, hipMemcpyHostToDevice));
+  // Waves per CU first (LDS tier on), then the sharing bound at the best
+  // occupancy, then the LDS tier off at the best pair.
   const int wpcs[] = {8, 12, 16, 20, 24};
-  const int taus[] = {8, 16, 24, 48, -1};
+  const int taus[] = {16, 24, 48, -1};
   const int base_wpc = p->waves_per_cu, base_tau = p->share_bound, base_lds = p->lds_tier;
   double best = 1e30, base_ms = 0;
   int best_wpc = base_wpc, best_tau = base_tau, best_lds = base_lds, trials = 0;
@@ -901,23 +903,30 @@ mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
   st = trial(base_wpc, base_tau, base_lds, ms);
   if (!st) st = trial(base_wpc, base_tau, base_lds, base_ms);
   if (!st) best = base_ms;
-  for (int lds = 1; lds >= 0 && !st; --lds)
-    for (int wpc : wpcs) {
-      if ((uint64_t)p->cus * wpc > p->max_waves || st) continue;
-      if (!st) st = trial(wpc, base_tau, lds, ms);
-      if (!st && ms < best) {
-        best = ms;
-        best_wpc = wpc;
-        best_lds = lds;
-        best_tau = base_tau;
-      }
+  for (int wpc : wpcs) {
+    if ((uint64_t)p->cus * wpc > p->max_waves || st || (wpc == base_wpc && base_lds == 1)) continue;
+    st = trial(wpc, base_tau, 1, ms);
+    if (!st && ms < best) {
+      best = ms;
+      best_wpc = wpc;
+      best_lds = 1;
+      best_tau = base_tau;
     }
+  }
   for (int tau : taus) {
     if (st) break;
+    if (tau == best_tau) continue;
     st = trial(best_wpc, tau, best_lds, ms);
     if (!st && ms < best) {
       best = ms;
       best_tau = tau;
+    }
+  }
+  if (!st && best_lds == 1) {
+    st = trial(best_wpc, best_tau, 0, ms);
+    if (!st && ms < best) {
+      best = ms;
+      best_lds = 0;
     }
   }
   (void)hipFree(d_sample);

@@ -108,7 +108,9 @@ struct KArgs {
   uint32_t g_hist_stride;
 };
 
-hipError_t launch_mce(const KArgs &a, uint32_t waves, hipStream_t stream);
+// waves_per_cu selects the kernel variant compiled for that occupancy
+// (<= 16: 4 waves per SIMD, <= 20: 5, <= 24: 6, else 8).
+hipError_t launch_mce(const KArgs &a, uint32_t waves, uint32_t waves_per_cu, hipStream_t stream);
 hipError_t launch_copy_probe(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
 // Zero-fills n 64-bit words with a kernel (safe inside captured hipGraphs,
 // unlike memset nodes).

@@ -300,7 +300,7 @@ __device__ __forceinline__ Lvl load_level(Ptr rec, const Sub &s) {
 // common neighbours of the moved vertices.  Writes the child's record to
 // `out` and returns it.
 template <class BmP, class GpP, class XxP, class OutP>
-__device__ Lvl make_child(const Sub &s, const Lvl &par, XxP par_xx, uint32_t c, OutP out, BmP bm, GpP gP,
+__device__ __forceinline__ Lvl make_child(const Sub &s, const Lvl &par, XxP par_xx, uint32_t c, OutP out, BmP bm, GpP gP,
                           Acc &acc) {
   const uint32_t l = lane(), W = s.W, rp = s.rows_pad, p = s.p;
   const bool wl = l < W;
@@ -399,7 +399,7 @@ __device__ __forceinline__ int claim_task(ShareRec *r, bool *last) {
 // reaches t (candidates of V below t moved from P to XP) - the same child
 // the publisher would have generated (BkPivotBitBalance.cuh:40-70).
 template <class BmP, class GpP, class SnapP, class OutP>
-__device__ Lvl materialize(const Sub &s, SnapP snap, uint32_t idx, OutP out, BmP bm, GpP gP, Acc &acc) {
+__device__ __forceinline__ Lvl materialize(const Sub &s, SnapP snap, uint32_t idx, OutP out, BmP bm, GpP gP, Acc &acc) {
   const uint32_t l = lane(), W = s.W;
   const bool wl = l < W;
   Lvl par = load_level(snap, s);
@@ -438,7 +438,7 @@ __device__ __forceinline__ void publish_bit(const KArgs &a, uint32_t w, bool set
 // into a frozen level but claims its remaining tasks like any other wave.
 // Returns the number of nodes this wave evaluated.
 template <class BmP, class GpP, class StP>
-__device__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm, GpP gP, StP stk, uint64_t *stk_g, Acc &acc,
+__device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm, GpP gP, StP stk, uint64_t *stk_g, Acc &acc,
                             unsigned long long *s_hist, uint32_t me, bool &published, bool eager = false,
                             bool hbm_pending = false) {
   constexpr bool kStkLds = !std::is_same<StP, uint64_t *>::value;
@@ -654,6 +654,44 @@ __device__ __forceinline__ void flush_graph(const KArgs &a, uint32_t g, Acc &acc
   __builtin_amdgcn_wave_barrier();
 }
 
+// Degeneracy of the graph on P0 (rows 0..p-1 of an LDS bitmap), p <= 256:
+// repeatedly removes a vertex of minimum remaining degree.  Every clique in
+// P0 has at most degeneracy + 1 vertices, which bounds the depth of the
+// search (the stack) more tightly than the maximum degree does.
+__device__ __forceinline__ uint32_t p0_degeneracy(const lds64 *T, uint32_t rp, uint32_t p, uint32_t W) {
+  const uint32_t l = lane();
+  uint32_t d[4];
+  bool al[4];
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j) {
+    const uint32_t u = j * 64 + l;
+    al[j] = u < p;
+    d[j] = 0;
+    if (al[j])
+      for (uint32_t k = 0; k < W; ++k) d[j] += (uint32_t)__popcll(T[(uint64_t)k * rp + u]);
+  }
+  uint32_t degen = 0;
+  for (uint32_t left = p; left > 0; --left) {
+    uint32_t key = ~0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      const uint32_t kj = (d[j] << 16) | (j * 64 + l);
+      if (al[j] && kj < key) key = kj;
+    }
+    const uint32_t m = wave_min_u32(key);
+    const uint32_t v = m & 0xFFFF, dv = m >> 16;
+    degen = dv > degen ? dv : degen;
+    if (dv + 1 >= left) break;  // the remaining vertices form a clique
+    const uint64_t kv = v >> 6, vb = 1ull << (v & 63);
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      if (j * 64 + l == v) al[j] = false;
+      if (al[j] && (T[kv * rp + j * 64 + l] & vb)) d[j]--;
+    }
+  }
+  return degen;
+}
+
 // Builds root `canon` and searches it.  Returns tree nodes evaluated by this
 // wave.
 //
@@ -667,7 +705,7 @@ __device__ __forceinline__ void flush_graph(const KArgs &a, uint32_t g, Acc &acc
 //   from N+(x) above v1.
 // The bitmap is built in this wave's LDS when it fits, else in its HBM data
 // region; an LDS-built bitmap is copied to HBM only when stealing needs it.
-__device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk_g, lds64 *lds, uint32_t me,
+__device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk_g, lds64 *lds, uint32_t me,
                                uint64_t canon, Acc &acc, unsigned long long *s_hist, bool &published,
                                bool eager) {
   const uint32_t l = lane();
@@ -858,8 +896,13 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk_g, l
     maxdeg = wave_max_u32(maxdeg);
   };
   build(in_lds ? (uint64_t *)lds : bm_g, in_lds ? (uint64_t *)(lds + nbw) : gp_g);
-  // Depth bound: a clique inside P0 has at most maxdeg + 1 vertices.
-  const uint64_t levels = (p < (uint64_t)maxdeg + 1 ? p : (uint64_t)maxdeg + 1) + 1;
+  // Depth bound: a clique inside P0 has at most maxdeg + 1 vertices, and at
+  // most degeneracy + 1 (computed when that can move the stack into LDS).
+  uint64_t levels = (p < (uint64_t)maxdeg + 1 ? p : (uint64_t)maxdeg + 1) + 1;
+  if (in_lds && p <= 256 && (nbw + p + levels * LS) * 8 > a.lds_bytes) {
+    const uint64_t dg = p0_degeneracy((const lds64 *)lds, rows_pad, (uint32_t)p, (uint32_t)W);
+    levels = (p < dg + 1 ? p : dg + 1) + 1;
+  }
   if (levels * LS * 8 > a.stack_bytes) {
     defer_root(a, canon, data_need, levels * LS * 8);
     return 0;
@@ -893,20 +936,19 @@ __device__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk_g, l
     return run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + nbw), st, stk_g, acc, s_hist, me, published,
                    eager, true);
   }
-  if (in_lds) {  // the stack does not fit next to it: search from HBM
-    for (uint64_t i = l; i < nbw; i += 64) bm_g[i] = lds[i];
-    for (uint64_t i = l; i < p; i += 64) gp_g[i] = lds[nbw + i];
-  }
   for (uint32_t j = l; j < Wx; j += 64) stk_g[4 * W + j] = low_mask(xq - (uint64_t)j * 64);
   store_rpx(stk_g, s, L0);
   wave_fence();
+  if (in_lds)  // bitmap and label hashes in LDS, the stack (too deep for LDS) in HBM
+    return run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + nbw), stk_g, stk_g, acc, s_hist, me, published,
+                   eager, true);
   return run_dfs(a, s, (const uint64_t *)bm_g, (const uint64_t *)gp_g, stk_g, stk_g, acc, s_hist, me, published,
                  eager);
 }
 
 // Looks for a published task (preferring waves on this XCD, which share
 // this wave's L2) and runs it.  Returns true when a task was run.
-__device__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk_g, lds64 *lds, Acc &acc,
+__device__ __forceinline__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk_g, lds64 *lds, Acc &acc,
                       unsigned long long *s_hist, bool &is_idle) {
   const uint32_t l = lane();
   const uint32_t nwords = (a.n_waves + 63) / 64;
@@ -948,13 +990,19 @@ __device__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk_g, lds64 *lds, 
   Sub s = r->sub;
   const uint64_t *snap = r->snap;
   if (l == 0) atomicAdd(&a.users[s.bm_owner], 1u);
+  // Tiers as for a root: bitmap, label hashes and stack in LDS; or bitmap
+  // and label hashes in LDS with the stack in HBM; or all in HBM.
+  const uint64_t nb = (uint64_t)s.W * s.rows_pad;
   const bool in_lds = a.lds_tier && lds_words(s) * 8 <= a.lds_bytes;
+  const bool bm_lds = a.lds_tier && (nb + s.p) * 8 <= a.lds_bytes;
   lds64 *st = nullptr;
-  if (in_lds) {
+  if (bm_lds) {
     st = stage_lds(s, lds);
     wave_fence();
-    materialize(s, snap, (uint32_t)idx, st, (const lds64 *)lds, (const lds64 *)(lds + (uint64_t)s.W * s.rows_pad),
-                acc);
+    if (in_lds)
+      materialize(s, snap, (uint32_t)idx, st, (const lds64 *)lds, (const lds64 *)(lds + nb), acc);
+    else
+      materialize(s, snap, (uint32_t)idx, stk_g, (const lds64 *)lds, (const lds64 *)(lds + nb), acc);
   } else {
     materialize(s, snap, (uint32_t)idx, stk_g, s.bm, s.gP, acc);
   }
@@ -963,8 +1011,10 @@ __device__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk_g, lds64 *lds, 
   bool pub_unused = false;
   uint64_t nodes;
   if (in_lds)
-    nodes = run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + (uint64_t)s.W * s.rows_pad), st, stk_g, acc,
-                    s_hist, me, pub_unused);
+    nodes = run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + nb), st, stk_g, acc, s_hist, me, pub_unused);
+  else if (bm_lds)
+    nodes = run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + nb), stk_g, stk_g, acc, s_hist, me,
+                    pub_unused);
   else
     nodes = run_dfs(a, s, s.bm, s.gP, stk_g, stk_g, acc, s_hist, me, pub_unused);
   if (a.root_graph) flush_graph(a, s.graph, acc, s_hist);
@@ -981,7 +1031,7 @@ __device__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk_g, lds64 *lds, 
 
 extern __shared__ uint64_t lds_dyn[];
 
-__global__ __launch_bounds__(64) void mce_kernel(KArgs a) {
+__device__ __forceinline__ void mce_main(const KArgs &a) {
   __shared__ unsigned long long s_hist[kLdsHist];
   const uint32_t l = lane();
   for (uint32_t i = l; i < kLdsHist; i += 64) s_hist[i] = 0;
@@ -1062,10 +1112,27 @@ __global__ __launch_bounds__(64) void mce_kernel(KArgs a) {
     if (s_hist[i]) atomicAdd(&a.hist[i], s_hist[i]);
 }
 
+// One persistent kernel per occupancy target: the register budget that
+// lets WPE waves share a SIMD (512 VGPRs / WPE).  The search is latency-bound
+// (dependent LDS and L2 round trips per tree node), so more resident waves
+// hide more latency, at the price of LDS per wave and a few spilled values.
+template <int WPE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void mce_kernel(KArgs a) {
+  mce_main(a);
+}
+
 }  // namespace
 
-hipError_t launch_mce(const KArgs &a, uint32_t waves, hipStream_t stream) {
-  hipLaunchKernelGGL(mce_kernel, dim3(waves), dim3(64), a.lds_tier ? a.lds_bytes : 0, stream, a);
+hipError_t launch_mce(const KArgs &a, uint32_t waves, uint32_t waves_per_cu, hipStream_t stream) {
+  const size_t lds = a.lds_tier ? a.lds_bytes : 0;
+  if (waves_per_cu <= 16)
+    hipLaunchKernelGGL(mce_kernel<4>, dim3(waves), dim3(64), lds, stream, a);
+  else if (waves_per_cu <= 20)
+    hipLaunchKernelGGL(mce_kernel<5>, dim3(waves), dim3(64), lds, stream, a);
+  else if (waves_per_cu <= 24)
+    hipLaunchKernelGGL(mce_kernel<6>, dim3(waves), dim3(64), lds, stream, a);
+  else
+    hipLaunchKernelGGL(mce_kernel<8>, dim3(waves), dim3(64), lds, stream, a);
   return hipGetLastError();
 }
 

@@ -176,3 +176,39 @@ def spearman(a, b) -> float:
     rb -= rb.mean()
     d = np.sqrt((ra * ra).sum() * (rb * rb).sum())
     return float((ra * rb).sum() / d) if d else 0.0
+
+
+@dataclasses.dataclass
+class FitReport:
+    sample_roots: int        # roots labelled by the sample run
+    sample_kernel_ms: float  # the sample run's kernel time
+    loss: float              # final training loss (mean squared error of log1p(cycles))
+    seconds: float           # wall time of labelling + features + training + prediction
+
+
+def fit_and_predict(graph, n_roots: int, stride: int = 16, epochs: int = 30, batch: int = 1024,
+                    lr: float = 3e-3, seed: int = 1, device: int = 0, **plan_kw):
+    """Learns per-root costs of the edge roots [0, n_roots) of `graph`:
+    every `stride`-th root is labelled with its measured cost (cycles) by
+    one plain run of that sample, a CostModel is trained on those labels by
+    the GPU kernels, and its predictions for all n_roots roots are returned
+    with the model and a FitReport."""
+    import time
+
+    from .solver import Plan
+
+    t0 = time.perf_counter()
+    sample = np.arange(0, n_roots, stride, dtype=np.uint64)
+    plan = Plan(graph, root_order=sample[::-1].copy(), want_root_stats=True, device=device, **plan_kw)
+    try:
+        out = plan.run()
+    finally:
+        plan.free()
+    F = root_features(graph, device)[:n_roots]
+    X, Y = training_set(F, out.root_cycles, sample.astype(np.int64), seed=seed)
+    model = CostModel(seed)
+    model.fit_normalizer(X)
+    loss = model.train(X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed, device=device)
+    pred = model.predict(F, device)
+    rep = FitReport(int(sample.size), float(out.kernel_ms), float(loss), time.perf_counter() - t0)
+    return pred, model, rep
