@@ -456,7 +456,7 @@ uint64_t worst_stack(uint64_t maxdeg, uint64_t maxout) {
 uint32_t lds_for(const mce_plan *p, int wpc, uint32_t override_bytes) {
   if (override_bytes) return std::min<uint32_t>(override_bytes, p->lds_per_block_max);
   uint64_t per_wave = p->lds_per_cu / (uint64_t)std::max(1, wpc);
-  per_wave = per_wave > 1024 ? per_wave - 1024 : 0;  // static histogram + slack
+  per_wave = per_wave > 1024 ? per_wave - 1024 : 0;  // the LDS histogram (kLdsHist words) + slack
   per_wave = std::min<uint64_t>(per_wave, p->lds_per_block_max);
   return (uint32_t)(per_wave / 256 * 256);
 }

@@ -30,6 +30,9 @@
 #include "mce_device.hpp"
 #include "rng.hpp"
 
+// Dynamic LDS of a wave (the launch gives its size).
+extern __shared__ uint64_t lds_dyn[];
+
 namespace rdmce {
 namespace {
 
@@ -206,7 +209,12 @@ __device__ __forceinline__ uint64_t bcast64(uint64_t v) {
          __builtin_amdgcn_readfirstlane((uint32_t)v);
 }
 
-__device__ __forceinline__ void emit(const KArgs &a, const Sub &s, Acc &acc, unsigned long long *s_hist,
+// LDS pointers (address space 3): the LDS tiers and the per-wave clique-size
+// histogram.
+using lds64 = __attribute__((address_space(3))) uint64_t;
+using ldsu64 = __attribute__((address_space(3))) unsigned long long;
+
+__device__ __forceinline__ void emit(const KArgs &a, const Sub &s, Acc &acc, ldsu64 *s_hist,
                                      uint32_t size, uint64_t key, uint64_t rk) {
   if (lane() == 0) {
     acc.count++;
@@ -221,30 +229,33 @@ __device__ __forceinline__ void emit(const KArgs &a, const Sub &s, Acc &acc, uns
       atomicOr(a.err, kErrHistOverflow);
     }
   }
-  if (a.list && lane() == 0) {
-    unsigned long long pos = atomicAdd(a.list_cursor, (unsigned long long)(size + 1));
+  if (a.list) {
+    // entry: size, the root's vertices, then R's members in P0 order; lane l
+    // writes the member of bit l of each word of R
+    const uint32_t l = lane();
+    unsigned long long pos = 0;
+    if (l == 0) pos = atomicAdd(a.list_cursor, (unsigned long long)(size + 1));
+    pos = bcast64(pos);
     if (pos + size + 1 > a.list_cap) {
-      atomicOr(a.err, kErrListFull);
+      if (l == 0) atomicOr(a.err, kErrListFull);
     } else {
       uint32_t *dst = a.list + pos;
-      uint32_t k = 0;
-      dst[k++] = size;
-      dst[k++] = a.labels[s.v0];
-      if (s.base_size == 2) dst[k++] = a.labels[s.v1];
+      if (l == 0) {
+        dst[0] = size;
+        dst[1] = a.labels[s.v0];
+        if (s.base_size == 2) dst[2] = a.labels[s.v1];
+      }
+      uint32_t k = 1 + s.base_size;
       for (uint32_t w = 0; w < s.W; ++w) {
-        uint64_t bits = readlane64(rk, w);
-        while (bits) {
-          uint32_t b = __builtin_ctzll(bits);
-          bits &= bits - 1;
-          dst[k++] = a.labels[s.Plist[w * 64 + b]];
-        }
+        const uint64_t bits = readlane64(rk, w);
+        if ((bits >> l) & 1) dst[k + rank_below(bits)] = a.labels[s.Plist[w * 64 + l]];
+        k += (uint32_t)__popcll(bits);
       }
     }
   }
 }
 
-// LDS pointers (address space 3) for the LDS-tier instantiations.
-using lds64 = __attribute__((address_space(3))) uint64_t;
+
 
 // Level record layout (u64 words): R[W] P[W] XP[W] V[W] XX[Wx] meta[4]
 //   meta[0] = key (sum of label hashes of the clique so far)
@@ -293,6 +304,19 @@ __device__ __forceinline__ Lvl load_level(Ptr rec, const Sub &s) {
   return L;
 }
 
+// |S \ N(u)| for this lane's bitmap row u when `act` (0 otherwise), with
+// the words of S held one per lane (lanes < W).  The broadcasts of S's words
+// are issued by the whole wave; only the row loads depend on `act`.
+template <class BmP>
+__device__ __forceinline__ uint32_t miss_count(BmP bm, uint32_t rp, uint32_t u, uint64_t sk, uint32_t W, bool act) {
+  uint32_t cnt = 0;
+  for (uint32_t k = 0; k < W; ++k) {
+    const uint64_t sw = readlane64(sk, k);
+    if (act) cnt += (uint32_t)__popcll(sw & ~(uint64_t)bm[(uint64_t)k * rp + u]);
+  }
+  return cnt;
+}
+
 // Child of candidate c (a vertex of P): R' = R + c, P' = P & N(c),
 // XP' = XP & N(c), XX' = rows of XX adjacent to c, followed by aggressive
 // node generation: every i in P' adjacent to all of P' \ {i} moves to R'
@@ -322,12 +346,8 @@ __device__ __forceinline__ Lvl make_child(const Sub &s, const Lvl &par, XxP par_
       uint64_t w = readlane64(ch.pk, j);
       if (w == 0) continue;
       acc.words += (uint64_t)__popcll(w) * W;
-      bool is = (w >> l) & 1;
-      uint32_t cnt = 0;
-      if (is) {
-        uint32_t i = j * 64 + l;
-        for (uint32_t k = 0; k < W; ++k) cnt += __popcll(readlane64(ch.pk, k) & ~bm[(uint64_t)k * rp + i]);
-      }
+      const bool is = (w >> l) & 1;
+      const uint32_t cnt = miss_count(bm, rp, j * 64 + l, ch.pk, W, is);
       uint64_t m = __ballot(is && cnt == 1);
       if (l == j) mk = m;
     }
@@ -356,14 +376,9 @@ __device__ __forceinline__ Lvl make_child(const Sub &s, const Lvl &par, XxP par_
     uint64_t nw = 0;
     if (w != 0) {
       acc.words += (uint64_t)__popcll(w) * (anym ? W + 1 : 1);
-      bool hit = false;
-      if ((w >> l) & 1) {
-        uint32_t u = p + j * 64 + l;
-        hit = (bm[(uint64_t)kc * rp + u] >> cb) & 1;
-        if (hit && anym) {
-          for (uint32_t k = 0; k < W && hit; ++k) hit = (readlane64(mk, k) & ~bm[(uint64_t)k * rp + u]) == 0;
-        }
-      }
+      const uint32_t u = p + j * 64 + l;
+      bool hit = ((w >> l) & 1) && ((bm[(uint64_t)kc * rp + u] >> cb) & 1);
+      if (anym) hit = miss_count(bm, rp, u, mk, W, hit) == 0 && hit;
       nw = __ballot(hit);
     }
     if (l == 0) out[4 * W + j] = nw;
@@ -439,7 +454,7 @@ __device__ __forceinline__ void publish_bit(const KArgs &a, uint32_t w, bool set
 // Returns the number of nodes this wave evaluated.
 template <class BmP, class GpP, class StP>
 __device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm, GpP gP, StP stk, uint64_t *stk_g, Acc &acc,
-                            unsigned long long *s_hist, uint32_t me, bool &published, bool eager = false,
+                            ldsu64 *s_hist, uint32_t me, bool &published, bool eager = false,
                             bool hbm_pending = false) {
   constexpr bool kStkLds = !std::is_same<StP, uint64_t *>::value;
   const uint32_t W = s.W, p = s.p, rp = s.rows_pad, LS = s.LS;
@@ -510,14 +525,12 @@ __device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm
         uint64_t cw = readlane64(L.pk | L.xpk, j);
         if (cw == 0) continue;
         acc.words += (uint64_t)__popcll(cw) * W;
-        if ((cw >> l) & 1) {
-          uint32_t u = j * 64 + l;
-          uint32_t cnt = 0;
-          for (uint32_t k = 0; k < W; ++k) cnt += __popcll(readlane64(L.pk, k) & ~bm[(uint64_t)k * rp + u]);
-          if (cnt < best_c) {
-            best_c = cnt;
-            best_u = u;
-          }
+        const bool act = (cw >> l) & 1;
+        const uint32_t u = j * 64 + l;
+        const uint32_t cnt = miss_count(bm, rp, u, L.pk, W, act);
+        if (act && cnt < best_c) {
+          best_c = cnt;
+          best_u = u;
         }
       }
       StP XX = rec + 4 * W;
@@ -525,14 +538,12 @@ __device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm
         uint64_t cw = XX[j];
         if (cw == 0) continue;
         acc.words += (uint64_t)__popcll(cw) * W;
-        if ((cw >> l) & 1) {
-          uint32_t u = p + j * 64 + l;
-          uint32_t cnt = 0;
-          for (uint32_t k = 0; k < W; ++k) cnt += __popcll(readlane64(L.pk, k) & ~bm[(uint64_t)k * rp + u]);
-          if (cnt < best_c) {
-            best_c = cnt;
-            best_u = u;
-          }
+        const bool act = (cw >> l) & 1;
+        const uint32_t u = p + j * 64 + l;
+        const uint32_t cnt = miss_count(bm, rp, u, L.pk, W, act);
+        if (act && cnt < best_c) {
+          best_c = cnt;
+          best_u = u;
         }
       }
       const uint32_t min_c = wave_min_u32(best_c);
@@ -635,8 +646,9 @@ __device__ __forceinline__ lds64 *stage_lds(const Sub &s, lds64 *lds) {
 
 // Batches: moves what this wave accumulated since the last flush (count,
 // hash, LDS histogram) to graph g's results.
-__device__ __forceinline__ void flush_graph(const KArgs &a, uint32_t g, Acc &acc, unsigned long long *s_hist) {
+__device__ __forceinline__ void flush_graph(const KArgs &a, uint32_t g, Acc &acc, ldsu64 *s_hist) {
   const uint32_t l = lane();
+  wave_fence();  // lane 0's histogram updates before every lane reads its entries
   if (l == 0) {
     if (acc.count) atomicAdd(&a.g_count[g], acc.count);
     if (acc.hash) atomicAdd(&a.g_hash[g], acc.hash);
@@ -706,7 +718,7 @@ __device__ __forceinline__ uint32_t p0_degeneracy(const lds64 *T, uint32_t rp, u
 // The bitmap is built in this wave's LDS when it fits, else in its HBM data
 // region; an LDS-built bitmap is copied to HBM only when stealing needs it.
 __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, uint64_t *stk_g, lds64 *lds, uint32_t me,
-                               uint64_t canon, Acc &acc, unsigned long long *s_hist, bool &published,
+                               uint64_t canon, Acc &acc, ldsu64 *s_hist, bool &published,
                                bool eager) {
   const uint32_t l = lane();
   const uint64_t desc = a.roots[canon];
@@ -949,7 +961,7 @@ __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, ui
 // Looks for a published task (preferring waves on this XCD, which share
 // this wave's L2) and runs it.  Returns true when a task was run.
 __device__ __forceinline__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk_g, lds64 *lds, Acc &acc,
-                      unsigned long long *s_hist, bool &is_idle) {
+                      ldsu64 *s_hist, bool &is_idle) {
   const uint32_t l = lane();
   const uint32_t nwords = (a.n_waves + 63) / 64;
   const uint64_t xcd = 0x0101010101010101ull << (me & 7);
@@ -1029,14 +1041,14 @@ __device__ __forceinline__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk
   return true;
 }
 
-extern __shared__ uint64_t lds_dyn[];
-
 __device__ __forceinline__ void mce_main(const KArgs &a) {
-  __shared__ unsigned long long s_hist[kLdsHist];
+  // The wave's LDS: the clique-size histogram of sizes < kLdsHist, then the
+  // area of the LDS tiers (lds_bytes).
+  ldsu64 *s_hist = (ldsu64 *)lds_dyn;
+  lds64 *lds = (lds64 *)lds_dyn + kLdsHist;
   const uint32_t l = lane();
   for (uint32_t i = l; i < kLdsHist; i += 64) s_hist[i] = 0;
   __syncthreads();
-  lds64 *lds = (lds64 *)lds_dyn;
   const uint32_t me = blockIdx.x;
   uint8_t *slot = a.slots + (uint64_t)me * a.slot_bytes;
   uint64_t *stk = (uint64_t *)(a.stacks + (uint64_t)me * a.stack_bytes);
@@ -1124,7 +1136,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, 8))) vo
 }  // namespace
 
 hipError_t launch_mce(const KArgs &a, uint32_t waves, uint32_t waves_per_cu, hipStream_t stream) {
-  const size_t lds = a.lds_tier ? a.lds_bytes : 0;
+  const size_t lds = kLdsHist * 8 + (a.lds_tier ? a.lds_bytes : 0);
   if (waves_per_cu <= 16)
     hipLaunchKernelGGL(mce_kernel<4>, dim3(waves), dim3(64), lds, stream, a);
   else if (waves_per_cu <= 20)

@@ -1,0 +1,119 @@
+"""Parity cases for the host emulation of the library (test infrastructure).
+
+Run with RDMCE_LIB naming the emulated build (tests/emu/_build/
+librdmce_emu.so); every case compares the kernels' results with the CPU
+oracle bit for bit (count, histogram, clique hash, per-root node counts and,
+where asked, the clique list).  The emulation runs one OS thread per lane,
+so the cases are small graphs and few waves (EMU_CUS compute units).
+
+    RDMCE_LIB=tests/emu/_build/librdmce_emu.so EMU_CUS=1 python tests/emu/run_cases.py [filter]
+"""
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TESTS = os.path.dirname(HERE)
+sys.path.insert(0, os.path.dirname(TESTS))
+sys.path.insert(0, TESTS)
+
+import numpy as np  # noqa: E402
+
+import rdmce_amd as R  # noqa: E402
+from graphs import closed_forms, random_graph  # noqa: E402
+from oracle_lib import OracleGraph, clique_hash  # noqa: E402
+
+
+def oracle_for(g, root_mode=2, root_nodes=True, list_cap=0):
+    og = OracleGraph.from_csr(g.GetRowOffset(), g.GetColIdx(), g.GetLabels(), g.GetDegeneracy())
+    return og.mce(root_mode=root_mode, root_nodes=root_nodes, list_cap=list_cap)
+
+
+def same(out, ref, lists=False, nodes=True):
+    if (out.count, out.hash, out.hist_dict()) != (ref.count, ref.hash, ref.hist):
+        return f"count/hash/hist {out.count} {out.hash} {out.hist_dict()} != {ref.count} {ref.hash} {ref.hist}"
+    if nodes and not np.array_equal(out.root_nodes, ref.root_nodes):
+        return "per-root node counts differ"
+    if lists:
+        if sorted(out.cliques) != sorted(tuple(sorted(c)) for c in ref.cliques):
+            return "clique lists differ"
+        if clique_hash(out.cliques) != out.hash:
+            return "clique list does not hash to the reported hash"
+    return None
+
+
+def graph(edges):
+    g = R.Graph.from_edges(edges)
+    g.SortByOrder(R.OrderType.DEG)
+    return g
+
+
+def cases():
+    wpc = 4
+    g = R.Graph()
+    g.LoadFromFile(os.path.join(TESTS, "golden", "zachary.txt"))
+    g.SortByOrder(R.OrderType.DEG)
+    yield "zachary list", lambda: same(R.solve(g, waves_per_cu=wpc, want_list=True, want_root_stats=True),
+                                       oracle_for(g, list_cap=1 << 16), lists=True)
+    for name, edges, _ in closed_forms():
+        if name in ("K130", "cocktail20", "moon_moser7"):
+            continue
+        cg = graph(edges)
+        yield f"closed {name}", lambda cg=cg: same(R.solve(cg, waves_per_cu=wpc, want_root_stats=True), oracle_for(cg))
+    for n, p, seed in [(60, 0.5, 1), (120, 0.3, 2), (45, 0.8, 3), (200, 0.08, 4)]:
+        rg = graph(random_graph(n, p, seed))
+        ref = oracle_for(rg)
+        for tau in (1, 4, 24, -1):
+            for lds in (1, 0):
+                yield (f"random G({n},{p}) tau={tau} lds={lds}",
+                       lambda rg=rg, ref=ref, tau=tau, lds=lds: same(
+                           R.solve(rg, waves_per_cu=wpc, share_bound=tau, lds_tier=lds, want_root_stats=True), ref))
+    vg = graph(random_graph(80, 0.4, 5))
+    yield "vertex roots", lambda: same(R.solve(vg, waves_per_cu=wpc, root_mode=R.ROOT_VERTEX, want_root_stats=True),
+                                       oracle_for(vg, root_mode=1))
+    dg = graph(random_graph(150, 0.35, 6))
+    yield "deferred pass", lambda: same(R.solve(dg, waves_per_cu=wpc, slot_bytes=4096, want_root_stats=True),
+                                        oracle_for(dg))
+    bgs = [graph(random_graph(30 + 10 * i, 0.3 + 0.05 * i, 10 + i)) for i in range(5)]
+
+    def batch():
+        b = R.Batch(bgs, waves_per_cu=wpc)
+        out = b.run()
+        b.free()
+        for i, bg in enumerate(bgs):
+            ref = oracle_for(bg, root_nodes=False)
+            if (int(out.count[i]), int(out.hash[i]), out.hist_dict(i)) != (ref.count, ref.hash, ref.hist):
+                return f"batch graph {i} differs"
+        return None
+    yield "batch", batch
+    pg = graph(random_graph(70, 0.5, 7))
+    pref = oracle_for(pg, root_nodes=False)
+
+    def replay():
+        plan = R.Plan(pg, waves_per_cu=wpc, use_graph=True)
+        for _ in range(3):
+            o = plan.run()
+            if (o.count, o.hash, o.hist_dict()) != (pref.count, pref.hash, pref.hist):
+                return "graph replay differs"
+        plan.free()
+        return None
+    yield "plan graph replay", replay
+
+
+def main():
+    flt = sys.argv[1] if len(sys.argv) > 1 else ""
+    fails = 0
+    for name, fn in cases():
+        if flt not in name:
+            continue
+        t = time.time()
+        print(f"[emu-case] {name} ...", end=" ", flush=True)
+        err = fn()
+        print(f"{'ok' if err is None else 'FAIL: ' + err} ({time.time() - t:.1f}s)", flush=True)
+        fails += err is not None
+    print(f"[emu] {fails} failed", flush=True)
+    sys.exit(1 if fails else 0)
+
+
+if __name__ == "__main__":
+    main()
