@@ -17,6 +17,8 @@ Workloads (synthetic, generated on the host):
                   clique count grows by >1e9 per core level (full enumeration
                   is out of reach for any implementation).
   er2048          Erdos-Renyi G(2048, 0.008), seed 1 (all roots).
+  cl4m-core300    Chung-Lu + planted communities, n = 4e6 (rdmce_amd.workloads).
+  batch10k-e1e6   6,315 small G(n, p) graphs in one launch (rdmce_amd.workloads).
 """
 from __future__ import annotations
 
@@ -41,16 +43,21 @@ def build_workload(name: str):
     return w.graph, w.plain_order(), w.desc
 
 
-def read_traffic(workload: str):
-    """HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE
-    (committed summary), or None."""
-    path = os.path.join(ROOT, "profiles", f"mce_kernel_{workload.replace('-', '_')}_pmc.md")
+def read_profile_value(workload: str, kind: str, key: str):
+    """A figure from a committed rocprofv3 summary (profiles/), or None:
+    kind "pmc" holds traffic_bytes_per_launch (FETCH_SIZE + WRITE_SIZE,
+    calibrated), kind "sq" valu_issue_frac and lds_bw_frac."""
+    path = os.path.join(ROOT, "profiles", f"mce_kernel_{workload.replace('-', '_')}_{kind}.md")
     try:
         with open(path) as f:
-            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+            m = re.search(key + r":\s*([0-9.eE+]+)", f.read())
             return float(m.group(1)) if m else None
     except OSError:
         return None
+
+
+def read_traffic(workload: str):
+    return read_profile_value(workload, "pmc", "traffic_bytes_per_launch")
 
 
 def cpu_baseline(g, ids, stride: int, threads: int):
@@ -94,6 +101,65 @@ def time_plan(plan, steps: int, warmup: int, label: str):
     return outs, wall
 
 
+def run_batch_workload(args):
+    """Batch workloads: all graphs in one persistent launch (a replayed
+    hipGraph) per step; the plain path solves one graph per call, as the
+    reference handles one graph per invocation (every 10th graph, timed)."""
+    import numpy as np
+    import rdmce_amd as R
+    from rdmce_amd import workloads
+
+    t = time.perf_counter()
+    bw = workloads.load_batch(args.workload)
+    prep_s = time.perf_counter() - t
+    print(f"[bench] {bw.desc} prepared in {prep_s:.1f}s", file=sys.stderr, flush=True)
+    golden = os.path.join(ROOT, "tests", "golden", "batch10k_s3_" + args.workload.split("-")[1] + ".json")
+    b = R.Batch(bw.graphs, use_graph=True)
+    for i in range(args.warmup):
+        o = b.run()
+        print(f"[bench] batch warmup {i}: {o.total_count} cliques, kernel {o.kernel_ms:.1f} ms", file=sys.stderr,
+              flush=True)
+    t0 = time.perf_counter()
+    outs = []
+    for i in range(args.steps):
+        outs.append(b.run())
+        print(f"[bench] batch step {i}: kernel {outs[-1].kernel_ms:.1f} ms", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    b.free()
+    first = outs[0]
+    for o in outs[1:]:
+        if not (np.array_equal(o.count, first.count) and np.array_equal(o.hash, first.hash)):
+            raise SystemExit("batch steps disagree")
+    digest = workloads.batch_digest(first.count, first.hash)
+    check = None
+    if os.path.exists(golden):
+        with open(golden) as f:
+            gd = json.load(f)
+        check = bool(gd["digest"] == digest and gd["total_count"] == first.total_count)
+        if not check:
+            raise SystemExit("batch results differ from the committed oracle golden")
+    # plain: one call per graph on every 10th graph
+    sample = bw.graphs[::10]
+    t = time.perf_counter()
+    sc = 0
+    for g in sample:
+        sc += R.solve(g).count
+    plain_wall = time.perf_counter() - t
+    value = first.total_count * args.steps / wall
+    line = {
+        "metric": METRIC, "value": value, "unit": "maximal cliques/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": dict(bw.desc, cliques=first.total_count, tree_nodes=first.total_nodes, digest=digest,
+                       golden_match=check, kernel_ms=sum(o.kernel_ms for o in outs) / len(outs),
+                       graph_replay=first.graph_replay, passes=first.passes, prep_s=round(prep_s, 2), mode="batch"),
+        "plain": {"value": sc / plain_wall, "sample": f"every 10th graph ({len(sample)} graphs, {sc} cliques), "
+                  f"one mce_solve call each, {plain_wall:.2f} s"},
+        "speedup_vs_plain": value / (sc / plain_wall),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,8 +172,11 @@ def main():
     ap.add_argument("--share-bound", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--tune-stride", type=int, default=1, help="autotune on every n-th root (1: all)")
+    ap.add_argument("--heavy-fraction", type=float, default=0.01,
+                    help="learned schedule: share of roots dispatched first by predicted cost (1: all)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=16)
+    ap.add_argument("--cpu-stride", type=int, default=2,
+                    help="time the CPU oracle on every n-th root of the workload (2: ~15 s on 16 threads)")
     ap.add_argument("--copy-probe", action="store_true", help="also launch the traffic calibration copy kernel")
     args = ap.parse_args()
 
@@ -116,6 +185,9 @@ def main():
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload.startswith("batch"):
+        return run_batch_workload(args)
 
     import numpy as np
     import rdmce_amd as R
@@ -142,7 +214,8 @@ def main():
     if args.mode == "tuned":
         t = time.perf_counter()
         pred, model, rep = learned.fit_and_predict(g, n_roots)
-        sched = learned.learned_schedule(pred, np.arange(n_roots, dtype=np.uint64))
+        sched = learned.learned_schedule(pred, np.arange(n_roots, dtype=np.uint64),
+                                         heavy_fraction=args.heavy_fraction)
         plan = R.Plan(g, root_order=sched.order, eager=sched.eager)
         tune = plan.autotune(stride=args.tune_stride)
         prep2 = time.perf_counter() - t
@@ -152,8 +225,10 @@ def main():
         o = outs[0]
         if (o.count, o.hash) != (plain.count, plain.hash) or not np.array_equal(o.hist, plain.hist):
             raise SystemExit("learned+tuned results differ from the plain path")
-        sched_info = {"schedule": "learned (longest predicted first)" if sched.learned else "plain (fallback)",
+        sched_info = {"schedule": "learned (predicted-heaviest first, then plain order)" if sched.learned
+                      else "plain (fallback)",
                       "learned": bool(sched.learned), "model_loss": rep.loss, "label_roots": rep.sample_roots,
+                      "heavy_fraction": args.heavy_fraction,
                       "waves_per_cu": tune["waves_per_cu"], "share_bound": tune["share_bound"],
                       "lds_tier": tune["lds_tier"], "tune_trials": tune["trials"],
                       "tune_stride": args.tune_stride, "learn_tune_s": round(prep2, 2)}
@@ -189,7 +264,9 @@ def main():
         "speedup_vs_plain": value / plain_info["value"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(args.workload),
-                     "algo_bytes": algo_bytes},
+                     "algo_bytes": algo_bytes,
+                     "valu_issue_frac": read_profile_value(args.workload, "sq", "valu_issue_frac"),
+                     "lds_bw_frac": read_profile_value(args.workload, "sq", "lds_bw_frac")},
     }
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)

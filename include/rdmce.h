@@ -151,7 +151,9 @@ typedef struct mce_result {
   uint64_t search_bytes;     /* bitmap bytes read by the search (8 per 64-bit word) */
   uint64_t build_bytes;      /* bytes moved building subproblems: both root adjacency
                                 lists, the candidate list and the bitmap */
-  uint64_t reserved[6];
+  uint64_t published_levels; /* levels exposed to other wavefronts */
+  uint64_t owner_tasks;      /* tasks of exposed levels that their publisher ran itself */
+  uint64_t reserved[4];
 } mce_result;
 
 /* BkSolverWrapper(Graph&, device_id), BkPivotBitBalance.cuh:611-678 and

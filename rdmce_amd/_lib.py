@@ -98,7 +98,9 @@ class MceResult(Structure):
         ("waves", c_int32),
         ("search_bytes", c_uint64),
         ("build_bytes", c_uint64),
-        ("reserved", c_uint64 * 6),
+        ("published_levels", c_uint64),
+        ("owner_tasks", c_uint64),
+        ("reserved", c_uint64 * 4),
     ]
 
 

@@ -406,7 +406,7 @@ struct mce_batch {
 
 namespace {
 enum Ctr { kCount, kHash, kNodes, kNext, kListCursor, kNDeferred, kDeferNeed, kDeferStackNeed, kShared, kActive,
-           kIdle, kErr, kWords, kBuild, kNumCtr };
+           kIdle, kErr, kWords, kBuild, kPublished, kOwnerTasks, kNumCtr };
 
 template <class T>
 hipError_t dalloc(T **p, uint64_t count) {
@@ -641,6 +641,8 @@ KArgs make_args(const mce_plan *p) {
   a.deferred_need = p->d_ctr + kDeferNeed;
   a.deferred_stack_need = p->d_ctr + kDeferStackNeed;
   a.shared_tasks = p->d_ctr + kShared;
+  a.published_levels = p->d_ctr + kPublished;
+  a.owner_tasks = p->d_ctr + kOwnerTasks;
   a.active = (unsigned int *)(p->d_ctr + kActive);
   a.idle = (unsigned int *)(p->d_ctr + kIdle);
   a.err = (unsigned int *)(p->d_ctr + kErr);
@@ -830,6 +832,8 @@ mce_status plan_run_impl(mce_plan *p, mce_result **out) {
   res->hash = ro.ctr[kHash];
   res->total_nodes = ro.ctr[kNodes];
   res->shared_tasks = ro.ctr[kShared];
+  res->published_levels = ro.ctr[kPublished];
+  res->owner_tasks = ro.ctr[kOwnerTasks];
   res->search_bytes = ro.ctr[kWords] * 8;
   res->build_bytes = ro.ctr[kBuild];
   if (p->d_list) {

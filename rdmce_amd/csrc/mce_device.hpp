@@ -85,6 +85,8 @@ struct KArgs {
   unsigned int *active;             // waves running a root or a task
   unsigned int *idle;               // waves looking for work
   unsigned long long *shared_tasks; // tasks taken by a wave other than the publisher
+  unsigned long long *published_levels;  // levels exposed
+  unsigned long long *owner_tasks;       // tasks of exposed levels run by their publisher
   int32_t share_bound;              // |P| threshold for exposing a level; <= 0: off
   uint32_t n_waves;
   // LDS tier: subproblems whose bitmap, label hashes and stack fit in

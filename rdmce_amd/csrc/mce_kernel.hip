@@ -504,6 +504,7 @@ __device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm
             __hip_atomic_store(&my->ctl, ((unsigned long long)ep << 32) | ((unsigned long long)n << 16),
                                __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             publish_bit(a, me, true);
+            atomicAdd(a.published_levels, 1ull);
           }
           published = true;
         }
@@ -590,6 +591,7 @@ __device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm
       idx = (int)bcast32((uint32_t)idx);
       if (idx >= 0) {
         if (bcast32(last) && l == 0) publish_bit(a, me, false);
+        if (l == 0) atomicAdd(a.owner_tasks, 1ull);
         level = S + 1;
         L = materialize(s, stk + (uint64_t)S * LS, (uint32_t)idx, stk + (uint64_t)level * LS, bm, gP, acc);
         if (l == 0) __hip_atomic_fetch_sub(&my->pending, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);

@@ -140,9 +140,13 @@ def plain_schedule(root_ids) -> Schedule:
     return Schedule(ids, np.zeros(ids.size, dtype=np.uint8), False)
 
 
-def learned_schedule(pred, root_ids, eager_top: int = 0) -> Schedule:
-    """Longest-predicted-first order of `root_ids` (canonical ids) from the
-    per-root predictions `pred` (indexed by canonical id); the `eager_top`
+def learned_schedule(pred, root_ids, eager_top: int = 0, heavy_fraction: float = 1.0) -> Schedule:
+    """Order of `root_ids` (canonical ids) from the per-root predictions
+    `pred` (indexed by canonical id): the predicted-heaviest `heavy_fraction`
+    of the roots first, longest predicted first, then the others in the
+    plain order (1.0: every root by prediction).  Keeping the bulk in the
+    plain order keeps its locality in the CSR (neighbouring roots share
+    adjacency rows) while the heavy tail starts early.  The `eager_top`
     heaviest are split at launch.  Falls back to the plain order when a
     prediction is not finite."""
     root_ids = np.asarray(root_ids, dtype=np.uint64)
@@ -152,7 +156,15 @@ def learned_schedule(pred, root_ids, eager_top: int = 0) -> Schedule:
     # stable: equal predictions keep the plain (decreasing id) order
     base = np.sort(root_ids)[::-1]
     pb = np.asarray(pred, dtype=np.float64)[base.astype(np.int64)]
-    order = base[np.argsort(-pb, kind="stable")].copy()
+    by_pred = np.argsort(-pb, kind="stable")
+    k = int(round(min(max(heavy_fraction, 0.0), 1.0) * base.size))
+    if k >= base.size:
+        order = base[by_pred].copy()
+    else:
+        heavy = by_pred[:k]
+        rest = np.ones(base.size, dtype=bool)
+        rest[heavy] = False
+        order = np.concatenate([base[heavy], base[rest]])
     eager = np.zeros(order.size, dtype=np.uint8)
     eager[:min(eager_top, order.size)] = 1
     return Schedule(order, eager, True)

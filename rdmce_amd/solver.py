@@ -41,6 +41,8 @@ class MceOutput:
     waves: int
     search_bytes: int
     build_bytes: int
+    published_levels: int = 0
+    owner_tasks: int = 0
 
     def hist_dict(self) -> dict:
         return {int(k): int(v) for k, v in enumerate(self.hist) if v}
@@ -94,7 +96,7 @@ def _convert(r: MceResult, want_list: bool, want_root_stats: bool) -> MceOutput:
         rc = np.ctypeslib.as_array(r.root_cycles, shape=(r.n_roots,)).copy()
     return MceOutput(int(r.count), int(r.hash), hist, cliques, rn, rc, int(r.total_nodes), int(r.deferred_roots),
                      int(r.shared_tasks), float(r.kernel_ms), float(r.total_ms), int(r.passes), int(r.waves),
-                     int(r.search_bytes), int(r.build_bytes))
+                     int(r.search_bytes), int(r.build_bytes), int(r.published_levels), int(r.owner_tasks))
 
 
 def solve(graph: Graph, **kw) -> MceOutput:

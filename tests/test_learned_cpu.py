@@ -46,3 +46,21 @@ def test_spearman():
     a = np.arange(50)
     assert learned.spearman(a, a * 3 + 1) > 0.999
     assert learned.spearman(a, -a) < -0.999
+
+
+def test_hybrid_schedule_heavy_first_then_plain():
+    """heavy_fraction: the predicted-heaviest share first (longest first),
+    the rest in the plain order; always a permutation of the roots."""
+    rng = np.random.default_rng(5)
+    ids = np.arange(1000, dtype=np.uint64)
+    pred = rng.normal(size=1000).astype(np.float32)
+    s = learned.learned_schedule(pred, ids, eager_top=3, heavy_fraction=0.05)
+    assert s.learned and np.array_equal(np.sort(s.order), ids)
+    top = s.order[:50].astype(np.int64)
+    assert np.all(np.diff(pred[top]) <= 0)
+    assert set(top) == set(np.argsort(-pred, kind="stable")[:50])
+    rest = s.order[50:].astype(np.int64)
+    assert np.all(np.diff(rest) < 0)  # plain order: decreasing canonical id
+    assert s.eager[:3].tolist() == [1, 1, 1] and s.eager[3:].sum() == 0
+    full = learned.learned_schedule(pred, ids, heavy_fraction=1.0)
+    assert np.all(np.diff(pred[full.order.astype(np.int64)]) <= 0)
