@@ -375,6 +375,7 @@ struct mce_plan {
   uint8_t *d_stacks = nullptr;
   ShareRec *d_recs = nullptr;
   unsigned int *d_users = nullptr;
+  unsigned int *d_idle = nullptr;  // kIdleCopies copies of the idle count, kIdleStride apart
   unsigned long long *d_has_work = nullptr;
   unsigned long long *d_ctr = nullptr;  // see Ctr
   unsigned long long *d_hist = nullptr;
@@ -428,7 +429,8 @@ void plan_release(mce_plan *p) {
                   p->d_perm,     p->d_eager,       p->d_slots,      p->d_ctr,         p->d_hist,
                   p->d_root_nodes, p->d_root_cycles, p->d_list,     p->d_deferred,    p->d_deferred2,
                   p->d_slots2,   p->d_stacks,      p->d_stacks2,    p->d_recs,        p->d_users,
-                  p->d_has_work, p->d_root_graph,  p->d_gcount,     p->d_ghash,       p->d_ghist};
+                  p->d_has_work, p->d_root_graph,  p->d_gcount,     p->d_ghash,       p->d_ghist,
+                  p->d_idle};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &e : p->ev)
@@ -595,6 +597,7 @@ mce_status plan_create_impl(const PlanSpec &spec, const mce_options &opt, mce_pl
   PTRY(dalloc(&p->d_recs, p->max_waves));
   PTRY(dalloc(&p->d_users, p->max_waves + 2));  // zeroed in whole 8-byte words
   PTRY(dalloc(&p->d_has_work, (p->max_waves + 63) / 64));
+  PTRY(dalloc(&p->d_idle, (uint64_t)kIdleCopies * kIdleStride));
   set_config(p, wpc, opt.share_bound == 0 ? 24 : opt.share_bound, opt.lds_tier ? 1 : 0);
 
   PTRY(dalloc(&p->d_ctr, kNumCtr));
@@ -644,7 +647,7 @@ KArgs make_args(const mce_plan *p) {
   a.published_levels = p->d_ctr + kPublished;
   a.owner_tasks = p->d_ctr + kOwnerTasks;
   a.active = (unsigned int *)(p->d_ctr + kActive);
-  a.idle = (unsigned int *)(p->d_ctr + kIdle);
+  a.idle_hint = p->d_idle;
   a.err = (unsigned int *)(p->d_ctr + kErr);
   a.hist = p->d_hist;
   a.hist_len = p->hist_len;
@@ -681,6 +684,7 @@ hipError_t enqueue_run(const mce_plan *p, const KArgs &a, hipStream_t s) {
   if (e == hipSuccess) e = launch_fill_zero(p->d_recs, sizeof(ShareRec) / 8 * p->waves, s);
   if (e == hipSuccess) e = launch_fill_zero(p->d_users, (p->waves + 1) / 2, s);
   if (e == hipSuccess) e = launch_fill_zero(p->d_has_work, (p->waves + 63) / 64, s);
+  if (e == hipSuccess) e = launch_fill_zero(p->d_idle, (uint64_t)kIdleCopies * kIdleStride / 2, s);
   if (e == hipSuccess && p->d_root_nodes) e = launch_fill_zero(p->d_root_nodes, p->n_roots, s);
   if (e == hipSuccess && p->d_root_cycles) e = launch_fill_zero(p->d_root_cycles, p->n_roots, s);
   if (e == hipSuccess && p->d_gcount) e = launch_fill_zero(p->d_gcount, p->n_graphs, s);
@@ -776,6 +780,7 @@ mce_status run_core(mce_plan *p, const KArgs &a0, bool allow_graph, RunOut &ro) 
     HIP_TRY(launch_fill_zero(p->d_recs, sizeof(ShareRec) / 8 * waves2, s));
     HIP_TRY(launch_fill_zero(p->d_users, (waves2 + 1) / 2, s));
     HIP_TRY(launch_fill_zero(p->d_has_work, (waves2 + 63) / 64, s));
+    HIP_TRY(launch_fill_zero(p->d_idle, (uint64_t)kIdleCopies * kIdleStride / 2, s));
     KArgs b = a;
     b.perm = p->d_deferred2;
     b.eager = nullptr;

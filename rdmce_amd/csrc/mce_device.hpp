@@ -10,6 +10,8 @@ constexpr uint32_t kMaxP = 4096;
 constexpr uint32_t kMaxW = kMaxP / 64;
 // Per-wave LDS histogram of clique sizes below this bound.
 constexpr uint32_t kLdsHist = 64;
+constexpr uint32_t kIdleCopies = 64;  // copies of the idle count, one per 128-byte line
+constexpr uint32_t kIdleStride = 32;  // u32 between copies
 
 // Error flags raised by the kernels (bit mask in KArgs::err).
 constexpr uint32_t kErrPTooLarge = 1u;       // a root has |P| > kMaxP
@@ -83,7 +85,7 @@ struct KArgs {
   unsigned int *users;              // per wave: other waves reading its data region
   unsigned long long *has_work;     // bit per wave: record has unclaimed tasks
   unsigned int *active;             // waves running a root or a task
-  unsigned int *idle;               // waves looking for work
+  unsigned int *idle_hint;          // kIdleCopies copies (kIdleStride apart) of the number of waves looking for work
   unsigned long long *shared_tasks; // tasks taken by a wave other than the publisher
   unsigned long long *published_levels;  // levels exposed
   unsigned long long *owner_tasks;       // tasks of exposed levels run by their publisher

@@ -442,6 +442,17 @@ __device__ __forceinline__ void publish_bit(const KArgs &a, uint32_t w, bool set
   else atomicAnd(&a.has_work[w >> 6], ~(1ull << (w & 63)));
 }
 
+// The number of waves looking for work is kept in kIdleCopies copies, one
+// per 128-byte line: a wave changing state updates every copy (one vector
+// atomic, lane i on copy i), and a searching wave polls only copy me % 64,
+// so the polls of thousands of waves do not queue on one address.
+__device__ __forceinline__ void idle_add(const KArgs &a, int d) {
+  atomicAdd(&a.idle_hint[lane() * kIdleStride], (unsigned int)d);
+}
+__device__ __forceinline__ bool any_idle(const KArgs &a, uint32_t me) {
+  return ld_relaxed(&a.idle_hint[(me % kIdleCopies) * kIdleStride]) != 0;
+}
+
 // Depth-first search from the record at level 0 of `stk` (a root or a
 // materialized task).  bm, gP and stk are either this wave's LDS copies or
 // HBM; stk_g is this wave's HBM stack arena (frozen levels are copied there
@@ -470,11 +481,11 @@ __device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm
   for (;;) {
     StP rec = stk + (uint64_t)level * LS;
     // An eager root (split before launch) exposes its top level at once;
-    // otherwise exposure waits for idle waves and is polled every 32 steps.
+    // otherwise exposure waits for idle waves and is polled every 64 steps.
     const bool eager_first = eager && S < 0;
-    if (sharing && S < Lstar && S + 1 < level && (eager_first || ((++tick & 31) == 0))) {
+    if (sharing && S < Lstar && S + 1 < level && (eager_first || ((++tick & 63) == 0))) {
       bool go = false;
-      if (l == 0) go = (eager_first || ld_relaxed(a.idle) > 0) && ld_relaxed(&my->pending) == 0;
+      if (l == 0) go = (eager_first || any_idle(a, me)) && ld_relaxed(&my->pending) == 0;
       if (bcast32(go)) {
         const uint32_t lp = (uint32_t)(S + 1);
         StP src = stk + (uint64_t)lp * LS;
@@ -995,12 +1006,12 @@ __device__ __forceinline__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk
   if (l == 0) idx = claim_task(r, &last);
   idx = (int)bcast32((uint32_t)idx);
   if (idx < 0) return false;
-  if (l == 0) {
-    if (last) publish_bit(a, (uint32_t)victim, false);
-    if (is_idle) atomicSub(a.idle, 1u);
-    atomicAdd(a.active, 1u);
+  if (l == 0 && last) publish_bit(a, (uint32_t)victim, false);
+  if (is_idle) {  // looking for work -> working
+    idle_add(a, -1);
+    if (l == 0) atomicAdd(a.active, 1u);
+    is_idle = false;
   }
-  is_idle = false;
   Sub s = r->sub;
   const uint64_t *snap = r->snap;
   if (l == 0) atomicAdd(&a.users[s.bm_owner], 1u);
@@ -1038,7 +1049,6 @@ __device__ __forceinline__ bool steal(const KArgs &a, uint32_t me, uint64_t *stk
     atomicAdd(a.total_nodes, (unsigned long long)nodes);
     atomicAdd(a.shared_tasks, 1ull);
     __hip_atomic_fetch_sub(&a.users[s.bm_owner], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    atomicSub(a.active, 1u);
   }
   return true;
 }
@@ -1056,9 +1066,17 @@ __device__ __forceinline__ void mce_main(const KArgs &a) {
   uint64_t *stk = (uint64_t *)(a.stacks + (uint64_t)me * a.stack_bytes);
   Acc acc{0, 0, 0, 0};
   unsigned long long nodes_total = 0;
+  // A wave is working (counted in `active`) until it finds neither a root
+  // nor a task; it is then looking for work (counted in the idle copies)
+  // until it gets either.  Only these transitions touch the shared
+  // counters, not every root.  The run ends for a wave that is looking for
+  // work when no roots are left and no wave is working: only working waves
+  // publish tasks, and a thief becomes working before the publisher it
+  // took from can finish.
   bool roots_left = true, is_idle = false;
   bool published = false;  // others may be reading this wave's data region
   uint32_t backoff = 1;
+  if (l == 0) atomicAdd(a.active, 1u);
   for (;;) {
     bool did = false;
     if (roots_left) {
@@ -1076,11 +1094,11 @@ __device__ __forceinline__ void mce_main(const KArgs &a) {
         if (l == 0) i = atomicAdd(a.next_root, 1ull);
         i = bcast64(i);
         if (i < a.n_dispatch) {
-          if (l == 0) {
-            if (is_idle) atomicSub(a.idle, 1u);
-            atomicAdd(a.active, 1u);
+          if (is_idle) {
+            idle_add(a, -1);
+            if (l == 0) atomicAdd(a.active, 1u);
+            is_idle = false;
           }
-          is_idle = false;
           uint64_t canon = a.perm ? a.perm[i] : (a.reverse ? a.n_dispatch - 1 - i : i);
           uint64_t t0 = clock64();
           const bool eager = a.eager && a.eager[i];
@@ -1090,7 +1108,6 @@ __device__ __forceinline__ void mce_main(const KArgs &a) {
           if (l == 0) {
             if (a.root_nodes && nodes) atomicAdd(&a.root_nodes[canon], (unsigned long long)nodes);
             if (a.root_cycles) atomicAdd(&a.root_cycles[canon], (unsigned long long)(clock64() - t0));
-            atomicSub(a.active, 1u);
           }
           did = true;
           backoff = 1;
@@ -1100,8 +1117,9 @@ __device__ __forceinline__ void mce_main(const KArgs &a) {
       }
     }
     if (did) continue;
-    if (!is_idle) {
-      if (l == 0) atomicAdd(a.idle, 1u);
+    if (!is_idle) {  // working -> looking for work
+      idle_add(a, 1);
+      if (l == 0) atomicSub(a.active, 1u);
       is_idle = true;
     }
     if (a.share_bound > 0 && steal(a, me, stk, lds, acc, s_hist, is_idle)) {
