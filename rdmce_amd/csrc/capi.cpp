@@ -880,7 +880,7 @@ mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
   if (!sample.empty()) HIP_TRY(hipMemcpy(d_sample, sample.data(), sample.size() * 8, hipMemcpyHostToDevice));
   // Waves per CU first (LDS tier on), then the sharing bound at the best
   // occupancy, then the LDS tier off at the best pair.
-  const int wpcs[] = {8, 12, 16, 20, 24};
+  const int wpcs[] = {8, 12, 16, 20, 24, 28, 32};
   const int taus[] = {16, 24, 48, -1};
   const int base_wpc = p->waves_per_cu, base_tau = p->share_bound, base_lds = p->lds_tier;
   double best = 1e30, base_ms = 0;

@@ -481,9 +481,11 @@ __device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm
   for (;;) {
     StP rec = stk + (uint64_t)level * LS;
     // An eager root (split before launch) exposes its top level at once;
-    // otherwise exposure waits for idle waves and is polled every 64 steps.
+    // otherwise exposure waits for idle waves, polled every kPollSteps steps
+    // (a poll is a round trip past this XCD's L2; a step is a few hundred
+    // cycles, and the tails worth splitting last milliseconds).
     const bool eager_first = eager && S < 0;
-    if (sharing && S < Lstar && S + 1 < level && (eager_first || ((++tick & 63) == 0))) {
+    if (sharing && S < Lstar && S + 1 < level && (eager_first || ((++tick & (kPollSteps - 1)) == 0))) {
       bool go = false;
       if (l == 0) go = (eager_first || any_idle(a, me)) && ld_relaxed(&my->pending) == 0;
       if (bcast32(go)) {
