@@ -60,6 +60,26 @@ def read_traffic(workload: str):
     return read_profile_value(workload, "pmc", "traffic_bytes_per_launch")
 
 
+def golden_check(workload: str, out):
+    """Compares a result with the committed oracle golden of `workload`
+    (tests/golden/*.json whose "workload" field names it): True on a
+    bit-exact match of count, hash and size histogram, None when no golden
+    exists; a mismatch stops the run."""
+    gdir = os.path.join(ROOT, "tests", "golden")
+    for fn in sorted(os.listdir(gdir)):
+        if not fn.endswith(".json"):
+            continue
+        with open(os.path.join(gdir, fn)) as f:
+            gd = json.load(f)
+        if not isinstance(gd, dict) or gd.get("workload") != workload:
+            continue
+        hist = {int(k): int(v) for k, v in gd["hist"].items()}
+        if (out.count, out.hash, out.hist_dict()) != (gd["count"], gd["hash"], hist):
+            raise SystemExit(f"{workload}: results differ from the committed oracle golden {fn}")
+        return True
+    return None
+
+
 def cpu_baseline(g, ids, stride: int, threads: int):
     """CPU oracle (multithreaded C port of the reference algorithm) on every
     `stride`-th root of the workload."""
@@ -234,6 +254,7 @@ def main():
                       "tune_stride": args.tune_stride, "learn_tune_s": round(prep2, 2)}
 
     first = outs[0]
+    golden_match = golden_check(args.workload, first)
     value = first.count * args.steps / wall
     kernel_ms = sum(o.kernel_ms for o in outs) / len(outs)
     algo_bytes = first.search_bytes + first.build_bytes
@@ -259,7 +280,8 @@ def main():
         "data": "synthetic",
         "config": dict(desc, cliques=first.count, clique_hash=first.hash, tree_nodes=first.total_nodes,
                        kernel_ms=kernel_ms, shared_tasks=first.shared_tasks, passes=first.passes,
-                       waves=first.waves, prep_s=round(prep_s, 2), mode=args.mode, **sched_info),
+                       waves=first.waves, golden_match=golden_match, prep_s=round(prep_s, 2), mode=args.mode,
+                       **sched_info),
         "plain": plain_info,
         "speedup_vs_plain": value / plain_info["value"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

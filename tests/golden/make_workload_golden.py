@@ -5,6 +5,7 @@ root; minutes on 8 host threads.
 
   python tests/golden/make_workload_golden.py rmat20-core499 tests/golden/rmat20_s2_core499.json
   python tests/golden/make_workload_golden.py cl4m-core300 tests/golden/cl4m_s4_core300.json
+  python tests/golden/make_workload_golden.py rmat24-core500 tests/golden/rmat24_s5_core500.json
   python tests/golden/make_workload_golden.py batch10k-e1e6 tests/golden/batch10k_s3_e1e6.json
 
 A batch golden holds the totals and a digest of the per-graph (count, hash)

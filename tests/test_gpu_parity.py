@@ -208,6 +208,27 @@ def test_cl4m_core300_full_size_golden():
     assert out.total_nodes == gd["tree_nodes"]
 
 
+def test_rmat24_core500_full_size_golden():
+    """R-MAT scale 24 (n = 8.9e6 after dropping isolated vertices, 2.6e8
+    edges), every edge root with core(v0) <= 500 (2.1e8 roots, 1.02e10
+    cliques, 2.3e10 search-tree nodes), against the committed CPU-oracle
+    golden (tests/golden/make_workload_golden.py; half an hour on 8 host
+    threads)."""
+    import json
+    from rdmce_amd import workloads
+    banner("rmat24 core<=500 full size")
+    with open(os.path.join(GOLDEN, "rmat24_s5_core500.json")) as f:
+        gd = json.load(f)
+    w = workloads.load("rmat24-core500")
+    assert (w.graph.GetNumVertices(), w.graph.GetNumEdges(), w.n_roots) == (gd["n"], gd["m"], gd["roots"])
+    banner(f"rmat24 graph built: {w.n_roots} roots")
+    out = R.solve(w.graph, root_order=w.plain_order())
+    assert out.count == gd["count"]
+    assert out.hash == gd["hash"]
+    assert out.hist_dict() == {int(k): v for k, v in gd["hist"].items()}
+    assert out.total_nodes == gd["tree_nodes"]
+
+
 def test_batch_per_graph_results():
     """Many graphs in one launch: per-graph count, hash and histogram equal
     each graph's own oracle result."""
