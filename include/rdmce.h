@@ -126,7 +126,10 @@ typedef struct mce_options {
   uint32_t lds_bytes;        /* LDS per wave for the LDS tier; 0 = CU LDS / waves_per_cu */
   const uint8_t *eager;      /* optional, one flag per dispatched root: expose its
                                 top level at once (a pre-launch split) */
-  uint64_t reserved[6];
+  int32_t word_path;         /* roots with |P0| <= 64 and |X0'| <= 64 search with their bitmap rows in
+                                registers: 0 = on (default), -1 = off (every root in the bitmap tiers) */
+  int32_t pad0;
+  uint64_t reserved[5];
 } mce_options;
 
 void mce_options_init(mce_options *opt);

@@ -45,7 +45,9 @@ class MceOptions(Structure):
         ("use_graph", c_int32),
         ("lds_bytes", c_uint32),
         ("eager", POINTER(ctypes.c_uint8)),
-        ("reserved", c_uint64 * 6),
+        ("word_path", c_int32),
+        ("pad0", c_int32),
+        ("reserved", c_uint64 * 5),
     ]
 
 

@@ -666,6 +666,7 @@ KArgs make_args(const mce_plan *p) {
   a.n_waves = p->waves;
   a.lds_tier = p->lds_tier;
   a.lds_bytes = p->lds_bytes;
+  a.word_path = p->opt.word_path >= 0;
   a.deferred = p->d_deferred;
   a.deferred_cap = p->deferred_cap;
   a.root_graph = p->d_root_graph;

@@ -633,6 +633,264 @@ __device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm
   return nodes;
 }
 
+// ---------------------------------------------------------------------------
+// Single-word path: |P0| <= 64 and |X0'| <= 64 (W == 1, Wx <= 1).  Each lane
+// keeps, for the whole root, the bitmap rows it is asked about: rowP = row
+// of P0 vertex l, rowX = row of X0' vertex l (bits over P0), gh = label hash
+// of P0 vertex l.  R, P, XP, V and XX are single uniform words, and a node
+// reads no memory:
+//   pivot   one masked popcount per lane and one wave min over
+//           |P \ N(u)| << 16 | u, the (count, lowest row) argmin of the scans
+//           of run_dfs;
+//   child   P' = P & N(c), XP' = XP & N(c); vertex i of P' moves to R' when
+//           P' & ~N(i) == {i} (one ballot); XP' and XX' keep the rows that
+//           contain every moved vertex (and c, for XX'), one ballot each -
+//           the bitmap is symmetric on P0, so x in N(m) iff m in N(x).
+// Level records keep the layout of the other tiers (R P XP V XX key meta:
+// words 0 .. 6 + Wx of LS = 8 + Wx), so publication, task claims and thieves
+// are unchanged.  A
+// record is written only when the search descends from its level (lane i
+// stores word i; later reads of it are by the same lanes), so leaves write
+// nothing.  Same search tree, node counts and accounting as run_dfs.
+// ---------------------------------------------------------------------------
+struct Lvl1 {
+  uint64_t r, p, x, v, xx, key;
+  uint32_t rsize;
+};
+
+// `v` (uniform) on lane ln, `old` elsewhere.
+__device__ __forceinline__ uint64_t writelane64(uint64_t old, uint64_t v, uint32_t ln) {
+  return lane() == ln ? v : old;
+}
+
+// Whole record of a level (lanes 0 .. 6 + Wx).  XX words: xlo = 0 and
+// xhi = 1 when XX is non-empty, else both 0 (as make_child writes them).
+template <class Ptr>
+__device__ __forceinline__ void store_lvl1(Ptr rec, uint32_t Wx, const Lvl1 &L) {
+  const uint32_t xc = (uint32_t)__popcll(L.xx);
+  uint64_t w = 0;
+  w = writelane64(w, L.r, 0);
+  w = writelane64(w, L.p, 1);
+  w = writelane64(w, L.x, 2);
+  w = writelane64(w, L.v, 3);
+  w = writelane64(w, L.xx, 4);  // the key's word when Wx == 0, overwritten below
+  w = writelane64(w, L.key, 4 + Wx);
+  w = writelane64(w, (uint64_t)L.rsize | ((uint64_t)xc << 32), 5 + Wx);
+  w = writelane64(w, xc ? (1ull << 32) : 0ull, 6 + Wx);
+  if (lane() < 7 + Wx) rec[lane()] = w;
+}
+
+// P, XP and V of a level whose record exists (lanes 1 .. 3).
+template <class Ptr>
+__device__ __forceinline__ void store_pxv1(Ptr rec, const Lvl1 &L) {
+  uint64_t w = 0;
+  w = writelane64(w, L.p, 1);
+  w = writelane64(w, L.x, 2);
+  w = writelane64(w, L.v, 3);
+  if (lane() - 1u < 3u) rec[lane()] = w;
+}
+
+template <class Ptr>
+__device__ __forceinline__ Lvl1 load_lvl1(Ptr rec, uint32_t Wx) {
+  const uint64_t w = lane() < 7 + Wx ? (uint64_t)rec[lane()] : 0ull;
+  Lvl1 L;
+  L.r = readlane64(w, 0);
+  L.p = readlane64(w, 1);
+  L.x = readlane64(w, 2);
+  L.v = readlane64(w, 3);
+  const uint64_t xw = readlane64(w, 4);
+  L.key = readlane64(w, 4 + Wx);
+  L.rsize = __builtin_amdgcn_readlane((uint32_t)w, 5 + Wx);
+  const uint32_t xhi = __builtin_amdgcn_readlane((uint32_t)(w >> 32), 6 + Wx);
+  L.xx = (Wx && xhi) ? xw : 0ull;  // a word outside [xlo, xhi) may be stale
+  return L;
+}
+
+// Child of candidate c, as make_child computes it.
+__device__ __forceinline__ Lvl1 child1(const Lvl1 &par, uint32_t c, uint64_t rowP, uint64_t rowX, uint64_t gh,
+                                       Acc &acc) {
+  const uint32_t l = lane();
+  const uint64_t rc = readlane64(rowP, c);
+  Lvl1 ch;
+  ch.r = par.r | (1ull << c);
+  ch.p = par.p & rc;
+  ch.x = par.x & rc;
+  ch.v = 0;
+  ch.key = par.key + readlane64(gh, c);
+  ch.rsize = par.rsize + 1;
+  acc.words += 1;
+  uint64_t mk = 0;
+  const uint32_t psz = (uint32_t)__popcll(ch.p);
+  if (psz > 1) {
+    acc.words += psz;
+    const bool is = (ch.p >> l) & 1;
+    mk = __ballot(is && __popcll(ch.p & ~rowP) == 1);
+    if (mk) {
+      ch.r |= mk;
+      ch.p &= ~mk;
+      ch.x &= __ballot((rowP & mk) == mk);
+      for (uint64_t m = mk; m; m &= m - 1) ch.key += readlane64(gh, (uint32_t)__builtin_ctzll(m));
+      const uint32_t nm = (uint32_t)__popcll(mk);
+      ch.rsize += nm;
+      acc.words += nm;
+    }
+  }
+  acc.words += (uint64_t)__popcll(par.xx) * (mk ? 2 : 1);
+  ch.xx = par.xx & __ballot(((rowX >> c) & 1) && (rowX & mk) == mk);
+  return ch;
+}
+
+template <class StP>
+__device__ __forceinline__ uint64_t run_dfs1(const KArgs &a, const Sub &s, const uint64_t *bm, const uint64_t *gP,
+                                             StP stk, uint64_t *stk_g, Acc &acc, ldsu64 *s_hist, uint32_t me,
+                                             bool &published, bool eager, bool hbm_pending) {
+  constexpr bool kStkLds = !std::is_same<StP, uint64_t *>::value;
+  const uint32_t p = s.p, rp = s.rows_pad, LS = s.LS, Wx = s.Wx;
+  const uint32_t l = lane();
+  const bool sharing = a.share_bound > 0;
+  ShareRec *my = a.recs + me;
+  const uint64_t rowP = l < p ? bm[l] : 0ull;
+  const uint64_t rowX = l < s.xq ? bm[p + l] : 0ull;
+  const uint64_t gh = l < p ? gP[l] : 0ull;
+  uint64_t nodes = 0;
+  Lvl1 L = load_lvl1(stk, Wx);
+  bool fresh = true;  // the current level's record is not written yet
+  int level = 0, st = 0;
+  int S = -1, Lstar = -1;
+  uint32_t tick = 0;
+  for (;;) {
+    const bool eager_first = eager && S < 0;
+    if (sharing && S < Lstar && S + 1 < level && (eager_first || ((++tick & (kPollSteps - 1)) == 0))) {
+      bool go = false;
+      if (l == 0) go = (eager_first || any_idle(a, me)) && ld_relaxed(&my->pending) == 0;
+      if (bcast32(go)) {
+        const uint32_t lp = (uint32_t)(S + 1);
+        StP src = stk + (uint64_t)lp * LS;
+        const uint64_t sw = l < LS ? (uint64_t)src[l] : 0ull;  // lane i wrote word i
+        const uint32_t n = (uint32_t)__popcll(readlane64(sw, 3));
+        if (n > 0) {
+          if (hbm_pending) {  // bitmap and label hashes exist only in LDS so far
+            uint64_t *dbm = (uint64_t *)s.bm, *dgp = (uint64_t *)s.gP;
+            for (uint64_t i = l; i < rp; i += 64) dbm[i] = bm[i];
+            for (uint64_t i = l; i < p; i += 64) dgp[i] = gP[i];
+            hbm_pending = false;
+          }
+          const uint64_t *snap_g;
+          if (kStkLds) {
+            uint64_t *dst = stk_g + (uint64_t)lp * LS;
+            if (l < LS) dst[l] = sw;
+            snap_g = dst;
+          } else {
+            snap_g = (const uint64_t *)src;
+          }
+          if (l == 0) {
+            __hip_atomic_store(&my->pending, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            my->snap = snap_g;
+            my->sub = s;
+            uint32_t ep = my->epoch + 1;
+            my->epoch = ep;
+            __hip_atomic_store(&my->ctl, ((unsigned long long)ep << 32) | ((unsigned long long)n << 16),
+                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            publish_bit(a, me, true);
+            atomicAdd(a.published_levels, 1ull);
+          }
+          published = true;
+        }
+        S = (int)lp;
+      }
+    }
+    if (st == 0) {
+      ++nodes;
+      const uint32_t psz = (uint32_t)__popcll(L.p);
+      if (psz == 0) {
+        if (L.x == 0 && L.xx == 0) emit(a, s, acc, s_hist, s.base_size + L.rsize, L.key, l == 0 ? L.r : 0ull);
+        st = 2;
+        continue;
+      }
+      const uint64_t cand = L.p | L.x;
+      acc.words += (uint64_t)__popcll(cand) + (uint64_t)__popcll(L.xx) + 1;
+      const uint32_t kp = ((cand >> l) & 1) ? ((uint32_t)__popcll(L.p & ~rowP) << 16) | l : ~0u;
+      const uint32_t kx = ((L.xx >> l) & 1) ? ((uint32_t)__popcll(L.p & ~rowX) << 16) | (p + l) : ~0u;
+      const uint32_t k = wave_min_u32(kp < kx ? kp : kx);
+      if ((k >> 16) == 0) {  // an excluded row covers P: no maximal clique below
+        st = 2;
+        continue;
+      }
+      const uint32_t piv = k & 0xFFFFu;
+      const uint64_t prow = piv < p ? readlane64(rowP, piv) : readlane64(rowX, piv - p);
+      L.v = L.p & ~prow;
+      if (sharing && ((int)psz >= a.share_bound || (eager && level == 0))) Lstar = level;
+      st = 1;
+      continue;
+    }
+    if (st == 1) {
+      if (L.v == 0) {
+        st = 2;
+        continue;
+      }
+      const uint32_t c = (uint32_t)__builtin_ctzll(L.v);
+      const uint64_t cbit = 1ull << c;
+      L.v &= ~cbit;
+      L.p &= ~cbit;
+      L.x |= cbit;
+      StP rec = stk + (uint64_t)level * LS;
+      if (fresh) store_lvl1(rec, Wx, L);
+      else store_pxv1(rec, L);
+      L = child1(L, c, rowP, rowX, gh, acc);
+      fresh = true;
+      ++level;
+      st = 0;
+      continue;
+    }
+    // st == 2: backtrack.
+    if (S >= 0 && level - 1 <= S) {
+      // The parent level is frozen: serve its remaining tasks.
+      int idx = -1;
+      bool last = false;
+      if (l == 0) idx = claim_task(my, &last);
+      idx = (int)bcast32((uint32_t)idx);
+      if (idx >= 0) {
+        if (bcast32(last) && l == 0) publish_bit(a, me, false);
+        if (l == 0) atomicAdd(a.owner_tasks, 1ull);
+        level = S + 1;
+        // candidate idx of the frozen V, with P and XP as sequential DFS
+        // holds them when it reaches it (as materialize does)
+        Lvl1 par = load_lvl1(stk + (uint64_t)S * LS, Wx);
+        uint64_t bits = par.v;
+        for (int k = 0; k < idx; ++k) bits &= bits - 1;
+        const uint32_t c = (uint32_t)__builtin_ctzll(bits);
+        const uint64_t below = par.v & ((1ull << c) - 1);
+        par.p &= ~below;
+        par.x |= below;
+        L = child1(par, c, rowP, rowX, gh, acc);
+        fresh = true;
+        if (l == 0) __hip_atomic_fetch_sub(&my->pending, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        Lstar = Lstar < S ? Lstar : S;
+        st = 0;
+        continue;
+      }
+      // All tasks taken; wait until the takers have copied their part of the
+      // frozen levels out.
+      if (l == 0) {
+        uint32_t n = 1;
+        while (ld_acquire(&my->pending) != 0) {
+          nap(n);
+          n = n < 8 ? n * 2 : 8;
+        }
+      }
+      wave_fence();
+      break;
+    }
+    if (level == 0) break;
+    --level;
+    L = load_lvl1(stk + (uint64_t)level * LS, Wx);
+    fresh = false;
+    Lstar = Lstar < level ? Lstar : level;
+    st = 1;
+  }
+  return nodes;
+}
+
 __device__ void defer_root(const KArgs &a, uint64_t canon, uint64_t data_need, uint64_t stack_need) {
   if (lane() == 0) {
     unsigned long long pos = atomicAdd(a.n_deferred, 1ull);
@@ -944,6 +1202,8 @@ __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, ui
   s.Plist = P;
   s.bm = bm_g;
   s.gP = gp_g;
+  // |P0| <= 64 and |X0'| <= 64: the single-word path (rows in registers).
+  const bool one_word = a.word_path && W == 1 && Wx <= 1;
   // Level 0: R = {}, P = P0, XP = {}, XX = X0'.
   Lvl L0;
   L0.rk = 0;
@@ -960,12 +1220,18 @@ __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, ui
     for (uint32_t j = l; j < Wx; j += 64) st[4 * W + j] = low_mask(xq - (uint64_t)j * 64);
     store_rpx(st, s, L0);
     wave_fence();
+    if (one_word)
+      return run_dfs1(a, s, (const uint64_t *)lds, (const uint64_t *)(lds + nbw), st, stk_g, acc, s_hist, me,
+                      published, eager, true);
     return run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + nbw), st, stk_g, acc, s_hist, me, published,
                    eager, true);
   }
   for (uint32_t j = l; j < Wx; j += 64) stk_g[4 * W + j] = low_mask(xq - (uint64_t)j * 64);
   store_rpx(stk_g, s, L0);
   wave_fence();
+  if (one_word)
+    return run_dfs1(a, s, in_lds ? (const uint64_t *)lds : bm_g, in_lds ? (const uint64_t *)(lds + nbw) : gp_g,
+                    stk_g, stk_g, acc, s_hist, me, published, eager, in_lds);
   if (in_lds)  // bitmap and label hashes in LDS, the stack (too deep for LDS) in HBM
     return run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + nbw), stk_g, stk_g, acc, s_hist, me, published,
                    eager, true);

@@ -68,6 +68,15 @@ def cases():
                 yield (f"random G({n},{p}) tau={tau} lds={lds}",
                        lambda rg=rg, ref=ref, tau=tau, lds=lds: same(
                            R.solve(rg, waves_per_cu=wpc, share_bound=tau, lds_tier=lds, want_root_stats=True), ref))
+    # bitmap in LDS with the stack in HBM (a small LDS area), and the bitmap
+    # tiers alone (word_path off)
+    mg = graph(random_graph(150, 0.45, 8))
+    mref = oracle_for(mg)
+    for lb in (1024, 2048):
+        yield (f"mixed tier lds_bytes={lb}",
+               lambda lb=lb: same(R.solve(mg, waves_per_cu=wpc, lds_bytes=lb, share_bound=2, want_root_stats=True),
+                                  mref))
+    yield "word path off", lambda: same(R.solve(mg, waves_per_cu=wpc, word_path=False, want_root_stats=True), mref)
     vg = graph(random_graph(80, 0.4, 5))
     yield "vertex roots", lambda: same(R.solve(vg, waves_per_cu=wpc, root_mode=R.ROOT_VERTEX, want_root_stats=True),
                                        oracle_for(vg, root_mode=1))
