@@ -740,95 +740,148 @@ __device__ __forceinline__ Lvl1 child1(const Lvl1 &par, uint32_t c, uint64_t row
   return ch;
 }
 
+// Evaluates node L: a leaf is emitted; otherwise the pivot is chosen and L.v
+// set to the candidates P \ N(pivot).  Returns |P| for a branching node, 0
+// for a leaf or a node whose P an excluded row covers (no candidates).
+__device__ __forceinline__ uint32_t eval1(const KArgs &a, const Sub &s, Lvl1 &L, uint64_t rowP, uint64_t rowX,
+                                          Acc &acc, ldsu64 *s_hist) {
+  const uint32_t l = lane(), p = s.p;
+  const uint32_t psz = (uint32_t)__popcll(L.p);
+  L.v = 0;
+  if (psz == 0) {
+    if (L.x == 0 && L.xx == 0) emit(a, s, acc, s_hist, s.base_size + L.rsize, L.key, l == 0 ? L.r : 0ull);
+    return 0;
+  }
+  const uint64_t cand = L.p | L.x;
+  acc.words += (uint64_t)__popcll(cand) + (uint64_t)__popcll(L.xx) + 1;
+  const uint32_t kp = ((cand >> l) & 1) ? ((uint32_t)__popcll(L.p & ~rowP) << 16) | l : ~0u;
+  const uint32_t kx = ((L.xx >> l) & 1) ? ((uint32_t)__popcll(L.p & ~rowX) << 16) | (p + l) : ~0u;
+  const uint32_t k = wave_min_u32(kp < kx ? kp : kx);
+  if ((k >> 16) == 0) return 0;  // an excluded row covers P: no maximal clique below
+  const uint32_t piv = k & 0xFFFFu;
+  const uint64_t prow = piv < p ? readlane64(rowP, piv) : readlane64(rowX, piv - p);
+  L.v = L.p & ~prow;
+  return psz;
+}
+
+// One pass of the loop per tree node: evaluate the node, backtrack to the
+// deepest level with a candidate left (or take a task of a frozen level),
+// descend.  Exposure of levels to idle waves (polled every kPollNodes
+// descents) and the serving of a frozen level's tasks follow run_dfs.
 template <class StP>
 __device__ __forceinline__ uint64_t run_dfs1(const KArgs &a, const Sub &s, const uint64_t *bm, const uint64_t *gP,
                                              StP stk, uint64_t *stk_g, Acc &acc, ldsu64 *s_hist, uint32_t me,
                                              bool &published, bool eager, bool hbm_pending) {
   constexpr bool kStkLds = !std::is_same<StP, uint64_t *>::value;
-  const uint32_t p = s.p, rp = s.rows_pad, LS = s.LS, Wx = s.Wx;
+  constexpr uint32_t kPollNodes = 64;
+  const uint32_t p = s.p, LS = s.LS, Wx = s.Wx;
   const uint32_t l = lane();
-  const bool sharing = a.share_bound > 0;
+  // Lstar (deepest level worth exposing) stays -1 without sharing, which
+  // keeps every exposure test false.
+  const int share_at = a.share_bound > 0 ? a.share_bound : 0x7fffffff;
+  const bool eager_top = eager && a.share_bound > 0;
   ShareRec *my = a.recs + me;
   const uint64_t rowP = l < p ? bm[l] : 0ull;
   const uint64_t rowX = l < s.xq ? bm[p + l] : 0ull;
   const uint64_t gh = l < p ? gP[l] : 0ull;
-  uint64_t nodes = 0;
   Lvl1 L = load_lvl1(stk, Wx);
-  bool fresh = true;  // the current level's record is not written yet
-  int level = 0, st = 0;
-  int S = -1, Lstar = -1;
+  int level = 0, S = -1, Lstar = -1;
   uint32_t tick = 0;
+  bool fresh = true;  // the current level's record is not written yet
+  uint64_t nodes = 0;
   for (;;) {
-    const bool eager_first = eager && S < 0;
-    if (sharing && S < Lstar && S + 1 < level && (eager_first || ((++tick & (kPollSteps - 1)) == 0))) {
-      bool go = false;
-      if (l == 0) go = (eager_first || any_idle(a, me)) && ld_relaxed(&my->pending) == 0;
-      if (bcast32(go)) {
-        const uint32_t lp = (uint32_t)(S + 1);
-        StP src = stk + (uint64_t)lp * LS;
-        const uint64_t sw = l < LS ? (uint64_t)src[l] : 0ull;  // lane i wrote word i
-        const uint32_t n = (uint32_t)__popcll(readlane64(sw, 3));
-        if (n > 0) {
-          if (hbm_pending) {  // bitmap and label hashes exist only in LDS so far
-            uint64_t *dbm = (uint64_t *)s.bm, *dgp = (uint64_t *)s.gP;
-            for (uint64_t i = l; i < rp; i += 64) dbm[i] = bm[i];
-            for (uint64_t i = l; i < p; i += 64) dgp[i] = gP[i];
-            hbm_pending = false;
-          }
-          const uint64_t *snap_g;
-          if (kStkLds) {
-            uint64_t *dst = stk_g + (uint64_t)lp * LS;
-            if (l < LS) dst[l] = sw;
-            snap_g = dst;
-          } else {
-            snap_g = (const uint64_t *)src;
-          }
+    ++nodes;
+    const uint32_t psz = eval1(a, s, L, rowP, rowX, acc, s_hist);
+    if ((int)psz >= share_at || (eager_top && level == 0 && psz > 0)) Lstar = level;
+    Lvl1 par;
+    uint32_t c;
+    bool task = false;
+    while (L.v == 0) {  // backtrack
+      if (S >= 0 && level - 1 <= S) {
+        // The parent level is frozen: serve its remaining tasks.
+        int idx = -1;
+        bool last = false;
+        if (l == 0) idx = claim_task(my, &last);
+        idx = (int)bcast32((uint32_t)idx);
+        if (idx < 0) {
+          // All tasks taken; wait until the takers have copied their part of
+          // the frozen levels out.
           if (l == 0) {
-            __hip_atomic_store(&my->pending, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            my->snap = snap_g;
-            my->sub = s;
-            uint32_t ep = my->epoch + 1;
-            my->epoch = ep;
-            __hip_atomic_store(&my->ctl, ((unsigned long long)ep << 32) | ((unsigned long long)n << 16),
-                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            publish_bit(a, me, true);
-            atomicAdd(a.published_levels, 1ull);
+            uint32_t n = 1;
+            while (ld_acquire(&my->pending) != 0) {
+              nap(n);
+              n = n < 8 ? n * 2 : 8;
+            }
           }
-          published = true;
+          wave_fence();
+          return nodes;
         }
-        S = (int)lp;
+        if (bcast32(last) && l == 0) publish_bit(a, me, false);
+        if (l == 0) atomicAdd(a.owner_tasks, 1ull);
+        // candidate idx of the frozen V, with P and XP as sequential DFS
+        // holds them when it reaches it (as materialize does)
+        par = load_lvl1(stk + (uint64_t)S * LS, Wx);
+        uint64_t bits = par.v;
+        for (int k = 0; k < idx; ++k) bits &= bits - 1;
+        c = (uint32_t)__builtin_ctzll(bits);
+        const uint64_t below = par.v & ((1ull << c) - 1);
+        par.p &= ~below;
+        par.x |= below;
+        if (l == 0) __hip_atomic_fetch_sub(&my->pending, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        Lstar = Lstar < S ? Lstar : S;
+        level = S;
+        task = true;
+        break;
       }
+      if (level == 0) return nodes;
+      --level;
+      L = load_lvl1(stk + (uint64_t)level * LS, Wx);
+      fresh = false;
+      Lstar = Lstar < level ? Lstar : level;
     }
-    if (st == 0) {
-      ++nodes;
-      const uint32_t psz = (uint32_t)__popcll(L.p);
-      if (psz == 0) {
-        if (L.x == 0 && L.xx == 0) emit(a, s, acc, s_hist, s.base_size + L.rsize, L.key, l == 0 ? L.r : 0ull);
-        st = 2;
-        continue;
+    if (!task) {
+      // Exposure: the shallowest level above this one that is not frozen.
+      if (S < Lstar && S + 1 < level && ((eager_top && S < 0) || ((++tick & (kPollNodes - 1)) == 0))) {
+        bool go = false;
+        if (l == 0) go = ((eager_top && S < 0) || any_idle(a, me)) && ld_relaxed(&my->pending) == 0;
+        if (bcast32(go)) {
+          const uint32_t lp = (uint32_t)(S + 1);
+          StP src = stk + (uint64_t)lp * LS;
+          const uint64_t sw = l < LS ? (uint64_t)src[l] : 0ull;  // lane i wrote word i
+          const uint32_t n = (uint32_t)__popcll(readlane64(sw, 3));
+          if (n > 0) {
+            if (hbm_pending) {  // bitmap and label hashes exist only in LDS so far
+              uint64_t *dbm = (uint64_t *)s.bm, *dgp = (uint64_t *)s.gP;
+              for (uint64_t i = l; i < s.rows_pad; i += 64) dbm[i] = bm[i];
+              for (uint64_t i = l; i < p; i += 64) dgp[i] = gP[i];
+              hbm_pending = false;
+            }
+            const uint64_t *snap_g;
+            if (kStkLds) {
+              uint64_t *dst = stk_g + (uint64_t)lp * LS;
+              if (l < LS) dst[l] = sw;
+              snap_g = dst;
+            } else {
+              snap_g = (const uint64_t *)src;
+            }
+            if (l == 0) {
+              __hip_atomic_store(&my->pending, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              my->snap = snap_g;
+              my->sub = s;
+              uint32_t ep = my->epoch + 1;
+              my->epoch = ep;
+              __hip_atomic_store(&my->ctl, ((unsigned long long)ep << 32) | ((unsigned long long)n << 16),
+                                 __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+              publish_bit(a, me, true);
+              atomicAdd(a.published_levels, 1ull);
+            }
+            published = true;
+          }
+          S = (int)lp;
+        }
       }
-      const uint64_t cand = L.p | L.x;
-      acc.words += (uint64_t)__popcll(cand) + (uint64_t)__popcll(L.xx) + 1;
-      const uint32_t kp = ((cand >> l) & 1) ? ((uint32_t)__popcll(L.p & ~rowP) << 16) | l : ~0u;
-      const uint32_t kx = ((L.xx >> l) & 1) ? ((uint32_t)__popcll(L.p & ~rowX) << 16) | (p + l) : ~0u;
-      const uint32_t k = wave_min_u32(kp < kx ? kp : kx);
-      if ((k >> 16) == 0) {  // an excluded row covers P: no maximal clique below
-        st = 2;
-        continue;
-      }
-      const uint32_t piv = k & 0xFFFFu;
-      const uint64_t prow = piv < p ? readlane64(rowP, piv) : readlane64(rowX, piv - p);
-      L.v = L.p & ~prow;
-      if (sharing && ((int)psz >= a.share_bound || (eager && level == 0))) Lstar = level;
-      st = 1;
-      continue;
-    }
-    if (st == 1) {
-      if (L.v == 0) {
-        st = 2;
-        continue;
-      }
-      const uint32_t c = (uint32_t)__builtin_ctzll(L.v);
+      // Descend into the lowest candidate.
+      c = (uint32_t)__builtin_ctzll(L.v);
       const uint64_t cbit = 1ull << c;
       L.v &= ~cbit;
       L.p &= ~cbit;
@@ -836,59 +889,12 @@ __device__ __forceinline__ uint64_t run_dfs1(const KArgs &a, const Sub &s, const
       StP rec = stk + (uint64_t)level * LS;
       if (fresh) store_lvl1(rec, Wx, L);
       else store_pxv1(rec, L);
-      L = child1(L, c, rowP, rowX, gh, acc);
-      fresh = true;
-      ++level;
-      st = 0;
-      continue;
+      par = L;
     }
-    // st == 2: backtrack.
-    if (S >= 0 && level - 1 <= S) {
-      // The parent level is frozen: serve its remaining tasks.
-      int idx = -1;
-      bool last = false;
-      if (l == 0) idx = claim_task(my, &last);
-      idx = (int)bcast32((uint32_t)idx);
-      if (idx >= 0) {
-        if (bcast32(last) && l == 0) publish_bit(a, me, false);
-        if (l == 0) atomicAdd(a.owner_tasks, 1ull);
-        level = S + 1;
-        // candidate idx of the frozen V, with P and XP as sequential DFS
-        // holds them when it reaches it (as materialize does)
-        Lvl1 par = load_lvl1(stk + (uint64_t)S * LS, Wx);
-        uint64_t bits = par.v;
-        for (int k = 0; k < idx; ++k) bits &= bits - 1;
-        const uint32_t c = (uint32_t)__builtin_ctzll(bits);
-        const uint64_t below = par.v & ((1ull << c) - 1);
-        par.p &= ~below;
-        par.x |= below;
-        L = child1(par, c, rowP, rowX, gh, acc);
-        fresh = true;
-        if (l == 0) __hip_atomic_fetch_sub(&my->pending, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        Lstar = Lstar < S ? Lstar : S;
-        st = 0;
-        continue;
-      }
-      // All tasks taken; wait until the takers have copied their part of the
-      // frozen levels out.
-      if (l == 0) {
-        uint32_t n = 1;
-        while (ld_acquire(&my->pending) != 0) {
-          nap(n);
-          n = n < 8 ? n * 2 : 8;
-        }
-      }
-      wave_fence();
-      break;
-    }
-    if (level == 0) break;
-    --level;
-    L = load_lvl1(stk + (uint64_t)level * LS, Wx);
-    fresh = false;
-    Lstar = Lstar < level ? Lstar : level;
-    st = 1;
+    L = child1(par, c, rowP, rowX, gh, acc);
+    fresh = true;
+    ++level;
   }
-  return nodes;
 }
 
 __device__ void defer_root(const KArgs &a, uint64_t canon, uint64_t data_need, uint64_t stack_need) {
