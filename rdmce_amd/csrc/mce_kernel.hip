@@ -453,10 +453,65 @@ __device__ __forceinline__ bool any_idle(const KArgs &a, uint32_t me) {
   return ld_relaxed(&a.idle_hint[(me % kIdleCopies) * kIdleStride]) != 0;
 }
 
+// Evaluates node L (the record at `rec`): a leaf is emitted; otherwise the
+// pivot is chosen and the candidates P \ N(pivot) set in L.vk and the
+// record.  Returns |P| for a branching node, 0 for a leaf or a node whose P
+// an excluded row covers (L.vk empty).
+template <class BmP, class GpP, class StP>
+__device__ __forceinline__ uint32_t eval_node(const KArgs &a, const Sub &s, Lvl &L, BmP bm, StP rec, Acc &acc,
+                                              ldsu64 *s_hist) {
+  const uint32_t W = s.W, p = s.p, rp = s.rows_pad;
+  const uint32_t l = lane();
+  const bool wl = l < W;
+  const uint32_t psz = set_popc(L.pk, W);
+  L.vk = 0;
+  if (psz == 0) {
+    if (__ballot(L.xpk != 0) == 0 && L.xxcnt == 0) emit(a, s, acc, s_hist, s.base_size + L.rsize, L.key, L.rk);
+    return 0;
+  }
+  // Pivot: argmin over candidate rows u of (|P \ N(u)|, u).  Each lane
+  // visits its rows in increasing u, so a strict < keeps its lowest u.
+  uint32_t best_c = ~0u, best_u = ~0u;
+  for (uint32_t j = 0; j < W; ++j) {
+    uint64_t cw = readlane64(L.pk | L.xpk, j);
+    if (cw == 0) continue;
+    acc.words += (uint64_t)__popcll(cw) * W;
+    const bool act = (cw >> l) & 1;
+    const uint32_t u = j * 64 + l;
+    const uint32_t cnt = miss_count(bm, rp, u, L.pk, W, act);
+    if (act && cnt < best_c) {
+      best_c = cnt;
+      best_u = u;
+    }
+  }
+  StP XX = rec + 4 * W;
+  for (uint32_t j = L.xlo; j < L.xhi; ++j) {
+    uint64_t cw = XX[j];
+    if (cw == 0) continue;
+    acc.words += (uint64_t)__popcll(cw) * W;
+    const bool act = (cw >> l) & 1;
+    const uint32_t u = p + j * 64 + l;
+    const uint32_t cnt = miss_count(bm, rp, u, L.pk, W, act);
+    if (act && cnt < best_c) {
+      best_c = cnt;
+      best_u = u;
+    }
+  }
+  const uint32_t min_c = wave_min_u32(best_c);
+  if (min_c == 0) return 0;  // an excluded row covers P: no maximal clique below
+  const uint32_t piv = wave_min_u32(best_c == min_c ? best_u : ~0u);
+  L.vk = wl ? (L.pk & ~bm[(uint64_t)l * rp + piv]) : 0;
+  acc.words += W;
+  if (wl) rec[3 * W + l] = L.vk;
+  return psz;
+}
+
 // Depth-first search from the record at level 0 of `stk` (a root or a
 // materialized task).  bm, gP and stk are either this wave's LDS copies or
 // HBM; stk_g is this wave's HBM stack arena (frozen levels are copied there
-// when the stack lives in LDS, so other waves can read them).
+// when the stack lives in LDS, so other waves can read them).  One pass of
+// the loop per tree node: evaluate the node, backtrack to the deepest level
+// with a candidate left (or take a task of a frozen level), descend.
 // Exposes levels root-down while other waves are idle
 // (BkPivotBitBalance.cuh:87-138): when a branching node has |P| >=
 // share_bound, the shallowest not yet exposed ancestor level is frozen and
@@ -471,23 +526,62 @@ __device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm
   const uint32_t W = s.W, p = s.p, rp = s.rows_pad, LS = s.LS;
   const uint32_t l = lane();
   const bool wl = l < W;
-  const bool sharing = a.share_bound > 0;
+  // Lstar (deepest level worth exposing) stays -1 without sharing, which
+  // keeps every exposure test false.
+  const int share_at = a.share_bound > 0 ? a.share_bound : 0x7fffffff;
+  const bool eager_top = eager && a.share_bound > 0;
   ShareRec *my = a.recs + me;
   uint64_t nodes = 0;
   Lvl L = load_level(stk, s);
-  int level = 0, st = 0;
+  int level = 0;
   int S = -1, Lstar = -1;  // levels <= S are frozen; Lstar: deepest level worth exposing
   uint32_t tick = 0;
   for (;;) {
-    StP rec = stk + (uint64_t)level * LS;
-    // An eager root (split before launch) exposes its top level at once;
-    // otherwise exposure waits for idle waves, polled every kPollSteps steps
-    // (a poll is a round trip past this XCD's L2; a step is a few hundred
-    // cycles, and the tails worth splitting last milliseconds).
-    const bool eager_first = eager && S < 0;
-    if (sharing && S < Lstar && S + 1 < level && (eager_first || ((++tick & (kPollSteps - 1)) == 0))) {
+    ++nodes;
+    const uint32_t psz = eval_node<BmP, GpP, StP>(a, s, L, bm, stk + (uint64_t)level * LS, acc, s_hist);
+    if ((int)psz >= share_at || (eager_top && level == 0 && psz > 0)) Lstar = level;
+    bool task = false;
+    while (__ballot(L.vk != 0) == 0) {  // backtrack
+      if (S >= 0 && level - 1 <= S) {
+        // The parent level is frozen: serve its remaining tasks.
+        int idx = -1;
+        bool last = false;
+        if (l == 0) idx = claim_task(my, &last);
+        idx = (int)bcast32((uint32_t)idx);
+        if (idx < 0) {
+          // All tasks taken; wait until the takers have copied their part of
+          // the frozen levels out.
+          if (l == 0) {
+            uint32_t n = 1;
+            while (ld_acquire(&my->pending) != 0) {
+              nap(n);
+              n = n < 8 ? n * 2 : 8;
+            }
+          }
+          wave_fence();
+          return nodes;
+        }
+        if (bcast32(last) && l == 0) publish_bit(a, me, false);
+        if (l == 0) atomicAdd(a.owner_tasks, 1ull);
+        level = S + 1;
+        L = materialize(s, stk + (uint64_t)S * LS, (uint32_t)idx, stk + (uint64_t)level * LS, bm, gP, acc);
+        if (l == 0) __hip_atomic_fetch_sub(&my->pending, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        Lstar = Lstar < S ? Lstar : S;
+        task = true;
+        break;
+      }
+      if (level == 0) return nodes;
+      --level;
+      L = load_level(stk + (uint64_t)level * LS, s);
+      Lstar = Lstar < level ? Lstar : level;
+    }
+    if (task) continue;
+    // Exposure (an eager root, split before launch, exposes its top level at
+    // once; otherwise exposure waits for idle waves, polled every kPollSteps
+    // descents - a poll is a round trip past this XCD's L2).
+    if (S < Lstar && S + 1 < level && ((eager_top && S < 0) || ((++tick & (kPollSteps - 1)) == 0))) {
       bool go = false;
-      if (l == 0) go = (eager_first || any_idle(a, me)) && ld_relaxed(&my->pending) == 0;
+      if (l == 0) go = ((eager_top && S < 0) || any_idle(a, me)) && ld_relaxed(&my->pending) == 0;
       if (bcast32(go)) {
         const uint32_t lp = (uint32_t)(S + 1);
         StP src = stk + (uint64_t)lp * LS;
@@ -524,113 +618,23 @@ __device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm
         S = (int)lp;
       }
     }
-    if (st == 0) {
-      ++nodes;
-      uint32_t psz = set_popc(L.pk, W);
-      if (psz == 0) {
-        if (__ballot(L.xpk != 0) == 0 && L.xxcnt == 0) emit(a, s, acc, s_hist, s.base_size + L.rsize, L.key, L.rk);
-        st = 2;
-        continue;
-      }
-      // Pivot: argmin over candidate rows u of (|P \ N(u)|, u).  Each lane
-      // visits its rows in increasing u, so a strict < keeps its lowest u.
-      uint32_t best_c = ~0u, best_u = ~0u;
-      for (uint32_t j = 0; j < W; ++j) {
-        uint64_t cw = readlane64(L.pk | L.xpk, j);
-        if (cw == 0) continue;
-        acc.words += (uint64_t)__popcll(cw) * W;
-        const bool act = (cw >> l) & 1;
-        const uint32_t u = j * 64 + l;
-        const uint32_t cnt = miss_count(bm, rp, u, L.pk, W, act);
-        if (act && cnt < best_c) {
-          best_c = cnt;
-          best_u = u;
-        }
-      }
-      StP XX = rec + 4 * W;
-      for (uint32_t j = L.xlo; j < L.xhi; ++j) {
-        uint64_t cw = XX[j];
-        if (cw == 0) continue;
-        acc.words += (uint64_t)__popcll(cw) * W;
-        const bool act = (cw >> l) & 1;
-        const uint32_t u = p + j * 64 + l;
-        const uint32_t cnt = miss_count(bm, rp, u, L.pk, W, act);
-        if (act && cnt < best_c) {
-          best_c = cnt;
-          best_u = u;
-        }
-      }
-      const uint32_t min_c = wave_min_u32(best_c);
-      if (min_c == 0) {  // an excluded row covers P: no maximal clique below
-        st = 2;
-        continue;
-      }
-      const uint32_t piv = wave_min_u32(best_c == min_c ? best_u : ~0u);
-      L.vk = wl ? (L.pk & ~bm[(uint64_t)l * rp + piv]) : 0;
-      acc.words += W;
-      if (wl) rec[3 * W + l] = L.vk;
-      if (sharing && ((int)psz >= a.share_bound || (eager && level == 0))) Lstar = level;
-      st = 1;
-      continue;
+    // Descend into the lowest candidate.
+    StP rec = stk + (uint64_t)level * LS;
+    const uint64_t vm = __ballot(L.vk != 0);
+    const uint32_t kc = (uint32_t)__builtin_ctzll(vm);
+    const uint32_t b = (uint32_t)__builtin_ctzll(readlane64(L.vk, kc));
+    const uint64_t cbit = 1ull << b;
+    if (l == kc) {
+      L.vk &= ~cbit;
+      L.pk &= ~cbit;
+      L.xpk |= cbit;
+      rec[W + l] = L.pk;
+      rec[2 * W + l] = L.xpk;
+      rec[3 * W + l] = L.vk;
     }
-    if (st == 1) {
-      uint64_t vm = __ballot(L.vk != 0);
-      if (vm == 0) {
-        st = 2;
-        continue;
-      }
-      const uint32_t kc = (uint32_t)__builtin_ctzll(vm);
-      const uint32_t b = (uint32_t)__builtin_ctzll(readlane64(L.vk, kc));
-      const uint64_t cbit = 1ull << b;
-      if (l == kc) {
-        L.vk &= ~cbit;
-        L.pk &= ~cbit;
-        L.xpk |= cbit;
-        rec[W + l] = L.pk;
-        rec[2 * W + l] = L.xpk;
-        rec[3 * W + l] = L.vk;
-      }
-      L = make_child(s, L, rec + 4 * W, kc * 64 + b, rec + LS, bm, gP, acc);
-      ++level;
-      st = 0;
-      continue;
-    }
-    // st == 2: backtrack.
-    if (S >= 0 && level - 1 <= S) {
-      // The parent level is frozen: serve its remaining tasks.
-      int idx = -1;
-      bool last = false;
-      if (l == 0) idx = claim_task(my, &last);
-      idx = (int)bcast32((uint32_t)idx);
-      if (idx >= 0) {
-        if (bcast32(last) && l == 0) publish_bit(a, me, false);
-        if (l == 0) atomicAdd(a.owner_tasks, 1ull);
-        level = S + 1;
-        L = materialize(s, stk + (uint64_t)S * LS, (uint32_t)idx, stk + (uint64_t)level * LS, bm, gP, acc);
-        if (l == 0) __hip_atomic_fetch_sub(&my->pending, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        Lstar = Lstar < S ? Lstar : S;
-        st = 0;
-        continue;
-      }
-      // All tasks taken; wait until the takers have copied their part of the
-      // frozen levels out.
-      if (l == 0) {
-        uint32_t n = 1;
-        while (ld_acquire(&my->pending) != 0) {
-          nap(n);
-          n = n < 8 ? n * 2 : 8;
-        }
-      }
-      wave_fence();
-      break;
-    }
-    if (level == 0) break;
-    --level;
-    L = load_level(stk + (uint64_t)level * LS, s);
-    Lstar = Lstar < level ? Lstar : level;
-    st = 1;
+    L = make_child(s, L, rec + 4 * W, kc * 64 + b, rec + LS, bm, gP, acc);
+    ++level;
   }
-  return nodes;
 }
 
 // ---------------------------------------------------------------------------
@@ -766,14 +770,13 @@ __device__ __forceinline__ uint32_t eval1(const KArgs &a, const Sub &s, Lvl1 &L,
 
 // One pass of the loop per tree node: evaluate the node, backtrack to the
 // deepest level with a candidate left (or take a task of a frozen level),
-// descend.  Exposure of levels to idle waves (polled every kPollNodes
+// descend.  Exposure of levels to idle waves (polled every kPollSteps
 // descents) and the serving of a frozen level's tasks follow run_dfs.
 template <class StP>
 __device__ __forceinline__ uint64_t run_dfs1(const KArgs &a, const Sub &s, const uint64_t *bm, const uint64_t *gP,
                                              StP stk, uint64_t *stk_g, Acc &acc, ldsu64 *s_hist, uint32_t me,
                                              bool &published, bool eager, bool hbm_pending) {
   constexpr bool kStkLds = !std::is_same<StP, uint64_t *>::value;
-  constexpr uint32_t kPollNodes = 64;
   const uint32_t p = s.p, LS = s.LS, Wx = s.Wx;
   const uint32_t l = lane();
   // Lstar (deepest level worth exposing) stays -1 without sharing, which
@@ -841,7 +844,7 @@ __device__ __forceinline__ uint64_t run_dfs1(const KArgs &a, const Sub &s, const
     }
     if (!task) {
       // Exposure: the shallowest level above this one that is not frozen.
-      if (S < Lstar && S + 1 < level && ((eager_top && S < 0) || ((++tick & (kPollNodes - 1)) == 0))) {
+      if (S < Lstar && S + 1 < level && ((eager_top && S < 0) || ((++tick & (kPollSteps - 1)) == 0))) {
         bool go = false;
         if (l == 0) go = ((eager_top && S < 0) || any_idle(a, me)) && ld_relaxed(&my->pending) == 0;
         if (bcast32(go)) {

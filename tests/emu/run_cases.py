@@ -80,8 +80,10 @@ def cases():
     # dense single-word roots: levels published, taken by thieves and served
     # by their owner
     sg = graph(random_graph(64, 0.85, 11))
-    yield "word path sharing", lambda: same(R.solve(sg, waves_per_cu=wpc, share_bound=1, want_root_stats=True),
-                                            oracle_for(sg))
+    sref = oracle_for(sg)
+    yield "word path sharing", lambda: same(R.solve(sg, waves_per_cu=wpc, share_bound=1, want_root_stats=True), sref)
+    yield "bitmap tier sharing", lambda: same(R.solve(sg, waves_per_cu=wpc, share_bound=1, word_path=False,
+                                                      want_root_stats=True), sref)
     vg = graph(random_graph(80, 0.4, 5))
     yield "vertex roots", lambda: same(R.solve(vg, waves_per_cu=wpc, root_mode=R.ROOT_VERTEX, want_root_stats=True),
                                        oracle_for(vg, root_mode=1))
