@@ -18,6 +18,8 @@ Workloads (synthetic, generated on the host):
                   is out of reach for any implementation).
   er2048          Erdos-Renyi G(2048, 0.008), seed 1 (all roots).
   cl4m-core300    Chung-Lu + planted communities, n = 4e6 (rdmce_amd.workloads).
+  rmat24-core500  R-MAT scale 24 (seed 5), roots with core(v0) <= 500
+                  (rdmce_amd.workloads).
   batch10k-e1e6   6,315 small G(n, p) graphs in one launch (rdmce_amd.workloads).
 """
 from __future__ import annotations
@@ -187,8 +189,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rmat20-core499")
     ap.add_argument("--mode", choices=["tuned", "plain"], default="tuned",
-                    help="tuned: time the plain path, then the learned schedule with autotuned settings "
-                         "(value); plain: the plain path only")
+                    help="tuned: time the plain path, then the autotuned settings with the learned schedule "
+                         "or, when one untimed run of each shows it faster, the plain order (value); "
+                         "plain: the plain path only")
     ap.add_argument("--share-bound", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--tune-stride", type=int, default=1, help="autotune on every n-th root (1: all)")
@@ -238,16 +241,29 @@ def main():
                                          heavy_fraction=args.heavy_fraction)
         plan = R.Plan(g, root_order=sched.order, eager=sched.eager)
         tune = plan.autotune(stride=args.tune_stride)
+        # The plain order with the tuned settings is the fallback: one untimed
+        # run of each order, and the faster one is timed.
+        alt = R.Plan(g, root_order=ids, waves_per_cu=tune["waves_per_cu"], share_bound=tune["share_bound"],
+                     lds_tier=tune["lds_tier"])
+        trial = {"learned": plan.run().kernel_ms, "plain": alt.run().kernel_ms}
+        use_learned = bool(sched.learned) and trial["learned"] <= trial["plain"]
+        if use_learned:
+            alt.free()
+        else:
+            plan.free()
+            plan = alt
         prep2 = time.perf_counter() - t
-        print(f"[bench] learned+tuned prep {prep2:.1f}s: fit {rep}, tune {tune}", file=sys.stderr, flush=True)
+        print(f"[bench] learned+tuned prep {prep2:.1f}s: fit {rep}, tune {tune}, order trial ms {trial}",
+              file=sys.stderr, flush=True)
         outs, wall = time_plan(plan, args.steps, args.warmup, "tuned")
         plan.free()
         o = outs[0]
         if (o.count, o.hash) != (plain.count, plain.hash) or not np.array_equal(o.hist, plain.hist):
             raise SystemExit("learned+tuned results differ from the plain path")
-        sched_info = {"schedule": "learned (predicted-heaviest first, then plain order)" if sched.learned
-                      else "plain (fallback)",
-                      "learned": bool(sched.learned), "model_loss": rep.loss, "label_roots": rep.sample_roots,
+        sched_info = {"schedule": "learned (predicted-heaviest first, then plain order)" if use_learned
+                      else "plain order with the tuned settings (measured faster than the learned order)",
+                      "learned": use_learned, "order_trial_ms": trial,
+                      "model_loss": rep.loss, "label_roots": rep.sample_roots,
                       "heavy_fraction": args.heavy_fraction,
                       "waves_per_cu": tune["waves_per_cu"], "share_bound": tune["share_bound"],
                       "lds_tier": tune["lds_tier"], "tune_trials": tune["trials"],
