@@ -126,7 +126,7 @@ typedef struct mce_options {
   uint32_t lds_bytes;        /* LDS per wave for the LDS tier; 0 = CU LDS / waves_per_cu */
   const uint8_t *eager;      /* optional, one flag per dispatched root: expose its
                                 top level at once (a pre-launch split) */
-  int32_t word_path;         /* roots with |P0| <= 64 and |X0'| <= 64 search with their bitmap rows in
+  int32_t word_path;         /* roots with |P0|, |X0'| <= 64 or <= 128 search with their bitmap rows in
                                 registers: 0 = on (default), -1 = off (every root in the bitmap tiers) */
   int32_t pad0;
   uint64_t reserved[5];

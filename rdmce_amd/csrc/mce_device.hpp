@@ -96,7 +96,7 @@ struct KArgs {
   // lds_bytes are searched entirely in the wave's LDS.
   int32_t lds_tier;
   uint32_t lds_bytes;
-  // Single-word path for roots with |P0| <= 64 and |X0'| <= 64.
+  // Register paths for roots with |P0|, |X0'| <= 64 or <= 128.
   int32_t word_path;
   // Roots whose scratch need exceeds the slots are deferred to a pass with
   // larger slots: canonical ids, count, and the largest needs seen.

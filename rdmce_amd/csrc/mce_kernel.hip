@@ -638,168 +638,296 @@ __device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm
 }
 
 // ---------------------------------------------------------------------------
-// Single-word path: |P0| <= 64 and |X0'| <= 64 (W == 1, Wx <= 1).  Each lane
-// keeps, for the whole root, the bitmap rows it is asked about: rowP = row
-// of P0 vertex l, rowX = row of X0' vertex l (bits over P0), gh = label hash
-// of P0 vertex l.  R, P, XP, V and XX are single uniform words, and a node
-// reads no memory:
-//   pivot   one masked popcount per lane and one wave min over
+// Register paths: |P0| <= 64 NW and |X0'| <= 64 NX (W == NW, Wx <= NX; NW,
+// NX = 1, 1 or 2, 1).  Each lane keeps, for the whole root, the bitmap rows
+// it is asked about: P[j][k] = word k of the row of P0 vertex 64 j + l,
+// X[j][k] = word k of the row of X0' vertex 64 j + l (bits over P0), g[j] =
+// label hash of P0 vertex 64 j + l.  R, P, XP and V are NW uniform words, XX
+// NX, and a node reads no memory:
+//   pivot   one masked popcount per lane and row slot and one wave min over
 //           |P \ N(u)| << 16 | u, the (count, lowest row) argmin of the scans
 //           of run_dfs;
 //   child   P' = P & N(c), XP' = XP & N(c); vertex i of P' moves to R' when
-//           P' & ~N(i) == {i} (one ballot); XP' and XX' keep the rows that
-//           contain every moved vertex (and c, for XX'), one ballot each -
-//           the bitmap is symmetric on P0, so x in N(m) iff m in N(x).
-// Level records keep the layout of the other tiers (R P XP V XX key meta:
-// words 0 .. 6 + Wx of LS = 8 + Wx), so publication, task claims and thieves
-// are unchanged.  A
-// record is written only when the search descends from its level (lane i
-// stores word i; later reads of it are by the same lanes), so leaves write
-// nothing.  Same search tree, node counts and accounting as run_dfs.
+//           P' & ~N(i) == {i} (one ballot per slot); XP' and XX' keep the
+//           rows that contain every moved vertex (and c, for XX') - the
+//           bitmap is symmetric on P0, so x in N(m) iff m in N(x).
+// Arrays are indexed by compile-time constants or through a uniform select
+// (no dynamic register indexing).  Level records keep the layout of the
+// other tiers (R P XP V XX key meta, words 0 .. 4 NW + Wx + 2 of LS), so
+// publication, task claims and thieves are unchanged.  A record is written
+// only when the search descends from its level (lane i stores word i;
+// later reads of it are by the same lanes), so leaves write nothing.  Same
+// search tree, node counts and accounting as run_dfs.
 // ---------------------------------------------------------------------------
-struct Lvl1 {
-  uint64_t r, p, x, v, xx, key;
+template <int NW, int NX>
+struct LvlR {
+  uint64_t r[NW], p[NW], x[NW], v[NW], xx[NX];
+  uint64_t key;
   uint32_t rsize;
 };
+
+template <int NW, int NX>
+struct RowsR {
+  uint64_t P[NW][NW], X[NX][NW], g[NW];
+};
+
+// Element j (uniform) of a per-slot array.
+template <int NW>
+__device__ __forceinline__ uint64_t pick(const uint64_t (&a)[NW], uint32_t j) {
+  if constexpr (NW == 1) {
+    return a[0];
+  } else {
+    return j ? a[1] : a[0];
+  }
+}
 
 // `v` (uniform) on lane ln, `old` elsewhere.
 __device__ __forceinline__ uint64_t writelane64(uint64_t old, uint64_t v, uint32_t ln) {
   return lane() == ln ? v : old;
 }
 
-// Whole record of a level (lanes 0 .. 6 + Wx).  XX words: xlo = 0 and
-// xhi = 1 when XX is non-empty, else both 0 (as make_child writes them).
-template <class Ptr>
-__device__ __forceinline__ void store_lvl1(Ptr rec, uint32_t Wx, const Lvl1 &L) {
-  const uint32_t xc = (uint32_t)__popcll(L.xx);
+// Whole record of a level.  XX words: [xlo, xhi) spans the non-zero ones
+// (both 0 when XX is empty), as make_child writes them.
+template <int NW, int NX, class Ptr>
+__device__ __forceinline__ void store_lvlr(Ptr rec, uint32_t Wx, const LvlR<NW, NX> &L) {
+  uint32_t xc = 0, xlo = 0, xhi = 0;
+#pragma unroll
+  for (int j = NX - 1; j >= 0; --j)
+    if (L.xx[j]) {
+      xc += (uint32_t)__popcll(L.xx[j]);
+      xlo = (uint32_t)j;
+      if (!xhi) xhi = (uint32_t)j + 1;
+    }
   uint64_t w = 0;
-  w = writelane64(w, L.r, 0);
-  w = writelane64(w, L.p, 1);
-  w = writelane64(w, L.x, 2);
-  w = writelane64(w, L.v, 3);
-  w = writelane64(w, L.xx, 4);  // the key's word when Wx == 0, overwritten below
-  w = writelane64(w, L.key, 4 + Wx);
-  w = writelane64(w, (uint64_t)L.rsize | ((uint64_t)xc << 32), 5 + Wx);
-  w = writelane64(w, xc ? (1ull << 32) : 0ull, 6 + Wx);
-  if (lane() < 7 + Wx) rec[lane()] = w;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    w = writelane64(w, L.r[k], k);
+    w = writelane64(w, L.p[k], NW + k);
+    w = writelane64(w, L.x[k], 2 * NW + k);
+    w = writelane64(w, L.v[k], 3 * NW + k);
+  }
+#pragma unroll
+  for (int j = 0; j < NX; ++j)
+    if ((uint32_t)j < Wx) w = writelane64(w, L.xx[j], 4 * NW + j);
+  w = writelane64(w, L.key, 4 * NW + Wx);
+  w = writelane64(w, (uint64_t)L.rsize | ((uint64_t)xc << 32), 4 * NW + Wx + 1);
+  w = writelane64(w, (uint64_t)xlo | ((uint64_t)xhi << 32), 4 * NW + Wx + 2);
+  if (lane() < 4 * NW + Wx + 3) rec[lane()] = w;
 }
 
-// P, XP and V of a level whose record exists (lanes 1 .. 3).
-template <class Ptr>
-__device__ __forceinline__ void store_pxv1(Ptr rec, const Lvl1 &L) {
+// P, XP and V of a level whose record exists (lanes NW .. 4 NW - 1).
+template <int NW, int NX, class Ptr>
+__device__ __forceinline__ void store_pxvr(Ptr rec, const LvlR<NW, NX> &L) {
   uint64_t w = 0;
-  w = writelane64(w, L.p, 1);
-  w = writelane64(w, L.x, 2);
-  w = writelane64(w, L.v, 3);
-  if (lane() - 1u < 3u) rec[lane()] = w;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    w = writelane64(w, L.p[k], NW + k);
+    w = writelane64(w, L.x[k], 2 * NW + k);
+    w = writelane64(w, L.v[k], 3 * NW + k);
+  }
+  if (lane() - (uint32_t)NW < 3u * NW) rec[lane()] = w;
 }
 
-template <class Ptr>
-__device__ __forceinline__ Lvl1 load_lvl1(Ptr rec, uint32_t Wx) {
-  const uint64_t w = lane() < 7 + Wx ? (uint64_t)rec[lane()] : 0ull;
-  Lvl1 L;
-  L.r = readlane64(w, 0);
-  L.p = readlane64(w, 1);
-  L.x = readlane64(w, 2);
-  L.v = readlane64(w, 3);
-  const uint64_t xw = readlane64(w, 4);
-  L.key = readlane64(w, 4 + Wx);
-  L.rsize = __builtin_amdgcn_readlane((uint32_t)w, 5 + Wx);
-  const uint32_t xhi = __builtin_amdgcn_readlane((uint32_t)(w >> 32), 6 + Wx);
-  L.xx = (Wx && xhi) ? xw : 0ull;  // a word outside [xlo, xhi) may be stale
+template <int NW, int NX, class Ptr>
+__device__ __forceinline__ LvlR<NW, NX> load_lvlr(Ptr rec, uint32_t Wx) {
+  const uint64_t w = lane() < 4 * NW + Wx + 3 ? (uint64_t)rec[lane()] : 0ull;
+  LvlR<NW, NX> L;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    L.r[k] = readlane64(w, k);
+    L.p[k] = readlane64(w, NW + k);
+    L.x[k] = readlane64(w, 2 * NW + k);
+    L.v[k] = readlane64(w, 3 * NW + k);
+  }
+  L.key = readlane64(w, 4 * NW + Wx);
+  L.rsize = __builtin_amdgcn_readlane((uint32_t)w, 4 * NW + Wx + 1);
+  const uint32_t xlo = __builtin_amdgcn_readlane((uint32_t)w, 4 * NW + Wx + 2);
+  const uint32_t xhi = __builtin_amdgcn_readlane((uint32_t)(w >> 32), 4 * NW + Wx + 2);
+#pragma unroll
+  for (int j = 0; j < NX; ++j) {  // a word outside [xlo, xhi) may be stale
+    const uint64_t xw = readlane64(w, 4 * NW + j);
+    L.xx[j] = ((uint32_t)j < Wx && (uint32_t)j >= xlo && (uint32_t)j < xhi) ? xw : 0ull;
+  }
   return L;
 }
 
 // Child of candidate c, as make_child computes it.
-__device__ __forceinline__ Lvl1 child1(const Lvl1 &par, uint32_t c, uint64_t rowP, uint64_t rowX, uint64_t gh,
-                                       Acc &acc) {
-  const uint32_t l = lane();
-  const uint64_t rc = readlane64(rowP, c);
-  Lvl1 ch;
-  ch.r = par.r | (1ull << c);
-  ch.p = par.p & rc;
-  ch.x = par.x & rc;
-  ch.v = 0;
-  ch.key = par.key + readlane64(gh, c);
+template <int NW, int NX>
+__device__ __forceinline__ LvlR<NW, NX> child_r(const LvlR<NW, NX> &par, uint32_t c, const RowsR<NW, NX> &rw,
+                                                Acc &acc) {
+  const uint32_t l = lane(), jc = c >> 6, bc = c & 63;
+  LvlR<NW, NX> ch;
+  uint32_t psz = 0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    const uint64_t rc = readlane64(jc ? rw.P[NW - 1][k] : rw.P[0][k], bc);
+    ch.r[k] = par.r[k] | ((uint32_t)k == jc ? 1ull << bc : 0ull);
+    ch.p[k] = par.p[k] & rc;
+    ch.x[k] = par.x[k] & rc;
+    ch.v[k] = 0;
+    psz += (uint32_t)__popcll(ch.p[k]);
+  }
+  ch.key = par.key + readlane64(pick<NW>(rw.g, jc), bc);
   ch.rsize = par.rsize + 1;
-  acc.words += 1;
-  uint64_t mk = 0;
-  const uint32_t psz = (uint32_t)__popcll(ch.p);
+  acc.words += NW;
+  uint64_t mk[NW];
+  bool anym = false;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) mk[k] = 0;
   if (psz > 1) {
-    acc.words += psz;
-    const bool is = (ch.p >> l) & 1;
-    mk = __ballot(is && __popcll(ch.p & ~rowP) == 1);
-    if (mk) {
-      ch.r |= mk;
-      ch.p &= ~mk;
-      ch.x &= __ballot((rowP & mk) == mk);
-      for (uint64_t m = mk; m; m &= m - 1) ch.key += readlane64(gh, (uint32_t)__builtin_ctzll(m));
-      const uint32_t nm = (uint32_t)__popcll(mk);
+    acc.words += (uint64_t)psz * NW;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      const bool is = (ch.p[j] >> l) & 1;
+      uint32_t cnt = 0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) cnt += (uint32_t)__popcll(ch.p[k] & ~rw.P[j][k]);
+      mk[j] = __ballot(is && cnt == 1);
+      anym = anym || mk[j] != 0;
+    }
+    if (anym) {
+      uint32_t nm = 0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        ch.r[k] |= mk[k];
+        ch.p[k] &= ~mk[k];
+        nm += (uint32_t)__popcll(mk[k]);
+      }
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        bool all = true;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) all = all && (rw.P[j][k] & mk[k]) == mk[k];
+        ch.x[j] &= __ballot(all);
+        for (uint64_t m = mk[j]; m; m &= m - 1) ch.key += readlane64(rw.g[j], (uint32_t)__builtin_ctzll(m));
+      }
       ch.rsize += nm;
-      acc.words += nm;
+      acc.words += (uint64_t)nm * NW;
     }
   }
-  acc.words += (uint64_t)__popcll(par.xx) * (mk ? 2 : 1);
-  ch.xx = par.xx & __ballot(((rowX >> c) & 1) && (rowX & mk) == mk);
+  uint32_t xc = 0;
+#pragma unroll
+  for (int j = 0; j < NX; ++j) xc += (uint32_t)__popcll(par.xx[j]);
+  acc.words += (uint64_t)xc * (anym ? NW + 1 : 1);
+#pragma unroll
+  for (int j = 0; j < NX; ++j) {
+    bool keep = (pick<NW>(rw.X[j], jc) >> bc) & 1;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) keep = keep && (rw.X[j][k] & mk[k]) == mk[k];
+    ch.xx[j] = par.xx[j] & __ballot(keep);
+  }
   return ch;
 }
 
 // Evaluates node L: a leaf is emitted; otherwise the pivot is chosen and L.v
 // set to the candidates P \ N(pivot).  Returns |P| for a branching node, 0
 // for a leaf or a node whose P an excluded row covers (no candidates).
-__device__ __forceinline__ uint32_t eval1(const KArgs &a, const Sub &s, Lvl1 &L, uint64_t rowP, uint64_t rowX,
-                                          Acc &acc, ldsu64 *s_hist) {
+template <int NW, int NX>
+__device__ __forceinline__ uint32_t eval_r(const KArgs &a, const Sub &s, LvlR<NW, NX> &L, const RowsR<NW, NX> &rw,
+                                           Acc &acc, ldsu64 *s_hist) {
   const uint32_t l = lane(), p = s.p;
-  const uint32_t psz = (uint32_t)__popcll(L.p);
-  L.v = 0;
+  uint32_t psz = 0;
+  uint64_t xany = 0, cw = 0, xw = 0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) {
+    psz += (uint32_t)__popcll(L.p[k]);
+    xany |= L.x[k];
+    cw += (uint64_t)__popcll(L.p[k] | L.x[k]);
+    L.v[k] = 0;
+  }
+#pragma unroll
+  for (int j = 0; j < NX; ++j) {
+    xany |= L.xx[j];
+    xw += (uint64_t)__popcll(L.xx[j]);
+  }
   if (psz == 0) {
-    if (L.x == 0 && L.xx == 0) emit(a, s, acc, s_hist, s.base_size + L.rsize, L.key, l == 0 ? L.r : 0ull);
+    if (xany == 0) {
+      uint64_t rk = 0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) rk = writelane64(rk, L.r[k], k);
+      emit(a, s, acc, s_hist, s.base_size + L.rsize, L.key, rk);
+    }
     return 0;
   }
-  const uint64_t cand = L.p | L.x;
-  acc.words += (uint64_t)__popcll(cand) + (uint64_t)__popcll(L.xx) + 1;
-  const uint32_t kp = ((cand >> l) & 1) ? ((uint32_t)__popcll(L.p & ~rowP) << 16) | l : ~0u;
-  const uint32_t kx = ((L.xx >> l) & 1) ? ((uint32_t)__popcll(L.p & ~rowX) << 16) | (p + l) : ~0u;
-  const uint32_t k = wave_min_u32(kp < kx ? kp : kx);
+  acc.words += (cw + xw + 1) * NW;
+  uint32_t best = ~0u;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) {
+    uint32_t cp = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) cp += (uint32_t)__popcll(L.p[k] & ~rw.P[j][k]);
+    const uint32_t kp = (((L.p[j] | L.x[j]) >> l) & 1) ? (cp << 16) | (64u * j + l) : ~0u;
+    best = kp < best ? kp : best;
+  }
+#pragma unroll
+  for (int j = 0; j < NX; ++j) {
+    uint32_t cx = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) cx += (uint32_t)__popcll(L.p[k] & ~rw.X[j][k]);
+    const uint32_t kx = ((L.xx[j] >> l) & 1) ? (cx << 16) | (p + 64u * j + l) : ~0u;
+    best = kx < best ? kx : best;
+  }
+  const uint32_t k = wave_min_u32(best);
   if ((k >> 16) == 0) return 0;  // an excluded row covers P: no maximal clique below
   const uint32_t piv = k & 0xFFFFu;
-  const uint64_t prow = piv < p ? readlane64(rowP, piv) : readlane64(rowX, piv - p);
-  L.v = L.p & ~prow;
+  const bool in_p = piv < p;
+  const uint32_t row = in_p ? piv : piv - p, js = row >> 6, ls = row & 63;
+#pragma unroll
+  for (int kk = 0; kk < NW; ++kk) {
+    const uint64_t pr = in_p ? (js ? rw.P[NW - 1][kk] : rw.P[0][kk]) : (js ? rw.X[NX - 1][kk] : rw.X[0][kk]);
+    L.v[kk] = L.p[kk] & ~readlane64(pr, ls);
+  }
   return psz;
 }
 
 // One pass of the loop per tree node: evaluate the node, backtrack to the
 // deepest level with a candidate left (or take a task of a frozen level),
 // descend.  Exposure of levels to idle waves (polled every kPollSteps
-// descents) and the serving of a frozen level's tasks follow run_dfs.
-template <class StP>
-__device__ __forceinline__ uint64_t run_dfs1(const KArgs &a, const Sub &s, const uint64_t *bm, const uint64_t *gP,
-                                             StP stk, uint64_t *stk_g, Acc &acc, ldsu64 *s_hist, uint32_t me,
-                                             bool &published, bool eager, bool hbm_pending) {
-  constexpr bool kStkLds = !std::is_same<StP, uint64_t *>::value;
-  const uint32_t p = s.p, LS = s.LS, Wx = s.Wx;
+// descents) and the serving of a frozen level's tasks follow run_dfs.  The
+// stack is addressed through a flat pointer (LDS or HBM, stk_lds telling
+// which): a node touches it at most once, and one instantiation per width
+// keeps the kernel's code and register allocation within budget.
+template <int NW, int NX>
+__device__ __forceinline__ uint64_t run_dfsr(const KArgs &a, const Sub &s, const uint64_t *bm, const uint64_t *gP,
+                                             uint64_t *stk, bool stk_lds, uint64_t *stk_g, Acc &acc, ldsu64 *s_hist,
+                                             uint32_t me, bool &published, bool eager, bool hbm_pending) {
+  const uint32_t p = s.p, rp = s.rows_pad, LS = s.LS, Wx = s.Wx;
   const uint32_t l = lane();
   // Lstar (deepest level worth exposing) stays -1 without sharing, which
   // keeps every exposure test false.
   const int share_at = a.share_bound > 0 ? a.share_bound : 0x7fffffff;
   const bool eager_top = eager && a.share_bound > 0;
   ShareRec *my = a.recs + me;
-  const uint64_t rowP = l < p ? bm[l] : 0ull;
-  const uint64_t rowX = l < s.xq ? bm[p + l] : 0ull;
-  const uint64_t gh = l < p ? gP[l] : 0ull;
-  Lvl1 L = load_lvl1(stk, Wx);
+  RowsR<NW, NX> rw;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) {
+    const uint32_t u = 64u * j + l;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) rw.P[j][k] = u < p ? bm[(uint64_t)k * rp + u] : 0ull;
+    rw.g[j] = u < p ? gP[u] : 0ull;
+  }
+#pragma unroll
+  for (int j = 0; j < NX; ++j) {
+    const uint32_t u = 64u * j + l;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) rw.X[j][k] = u < s.xq ? bm[(uint64_t)k * rp + p + u] : 0ull;
+  }
+  LvlR<NW, NX> L = load_lvlr<NW, NX>(stk, Wx);
   int level = 0, S = -1, Lstar = -1;
   uint32_t tick = 0;
   bool fresh = true;  // the current level's record is not written yet
   uint64_t nodes = 0;
   for (;;) {
     ++nodes;
-    const uint32_t psz = eval1(a, s, L, rowP, rowX, acc, s_hist);
+    const uint32_t psz = eval_r<NW, NX>(a, s, L, rw, acc, s_hist);
     if ((int)psz >= share_at || (eager_top && level == 0 && psz > 0)) Lstar = level;
-    Lvl1 par;
     uint32_t c;
     bool task = false;
-    while (L.v == 0) {  // backtrack
+    for (;;) {  // backtrack while no candidate is left
+      uint64_t vany = 0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) vany |= L.v[k];
+      if (vany) break;
       if (S >= 0 && level - 1 <= S) {
         // The parent level is frozen: serve its remaining tasks.
         int idx = -1;
@@ -823,13 +951,25 @@ __device__ __forceinline__ uint64_t run_dfs1(const KArgs &a, const Sub &s, const
         if (l == 0) atomicAdd(a.owner_tasks, 1ull);
         // candidate idx of the frozen V, with P and XP as sequential DFS
         // holds them when it reaches it (as materialize does)
-        par = load_lvl1(stk + (uint64_t)S * LS, Wx);
-        uint64_t bits = par.v;
-        for (int k = 0; k < idx; ++k) bits &= bits - 1;
-        c = (uint32_t)__builtin_ctzll(bits);
-        const uint64_t below = par.v & ((1ull << c) - 1);
-        par.p &= ~below;
-        par.x |= below;
+        L = load_lvlr<NW, NX>(stk + (uint64_t)S * LS, Wx);
+        uint32_t before = 0, jt = 0;
+        if constexpr (NW > 1) {
+          const uint32_t n0 = (uint32_t)__popcll(L.v[0]);
+          if ((uint32_t)idx >= n0) {
+            jt = 1;
+            before = n0;
+          }
+        }
+        uint64_t bits = pick<NW>(L.v, jt);
+        for (uint32_t k = before; k < (uint32_t)idx; ++k) bits &= bits - 1;
+        const uint32_t bt = (uint32_t)__builtin_ctzll(bits);
+        c = 64u * jt + bt;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+          const uint64_t below = (uint32_t)k < jt ? L.v[k] : ((uint32_t)k == jt ? L.v[k] & ((1ull << bt) - 1) : 0ull);
+          L.p[k] &= ~below;
+          L.x[k] |= below;
+        }
         if (l == 0) __hip_atomic_fetch_sub(&my->pending, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         Lstar = Lstar < S ? Lstar : S;
         level = S;
@@ -838,7 +978,7 @@ __device__ __forceinline__ uint64_t run_dfs1(const KArgs &a, const Sub &s, const
       }
       if (level == 0) return nodes;
       --level;
-      L = load_lvl1(stk + (uint64_t)level * LS, Wx);
+      L = load_lvlr<NW, NX>(stk + (uint64_t)level * LS, Wx);
       fresh = false;
       Lstar = Lstar < level ? Lstar : level;
     }
@@ -849,23 +989,24 @@ __device__ __forceinline__ uint64_t run_dfs1(const KArgs &a, const Sub &s, const
         if (l == 0) go = ((eager_top && S < 0) || any_idle(a, me)) && ld_relaxed(&my->pending) == 0;
         if (bcast32(go)) {
           const uint32_t lp = (uint32_t)(S + 1);
-          StP src = stk + (uint64_t)lp * LS;
-          const uint64_t sw = l < LS ? (uint64_t)src[l] : 0ull;  // lane i wrote word i
-          const uint32_t n = (uint32_t)__popcll(readlane64(sw, 3));
+          const uint64_t *src = stk + (uint64_t)lp * LS;
+          const uint64_t sw = l < LS ? src[l] : 0ull;  // lane i wrote word i
+          uint32_t n = 0;
+#pragma unroll
+          for (int k = 0; k < NW; ++k) n += (uint32_t)__popcll(readlane64(sw, 3 * NW + k));
           if (n > 0) {
             if (hbm_pending) {  // bitmap and label hashes exist only in LDS so far
               uint64_t *dbm = (uint64_t *)s.bm, *dgp = (uint64_t *)s.gP;
-              for (uint64_t i = l; i < s.rows_pad; i += 64) dbm[i] = bm[i];
+              const uint64_t nb = (uint64_t)NW * rp;
+              for (uint64_t i = l; i < nb; i += 64) dbm[i] = bm[i];
               for (uint64_t i = l; i < p; i += 64) dgp[i] = gP[i];
               hbm_pending = false;
             }
-            const uint64_t *snap_g;
-            if (kStkLds) {
+            const uint64_t *snap_g = src;
+            if (stk_lds) {
               uint64_t *dst = stk_g + (uint64_t)lp * LS;
               if (l < LS) dst[l] = sw;
               snap_g = dst;
-            } else {
-              snap_g = (const uint64_t *)src;
             }
             if (l == 0) {
               __hip_atomic_store(&my->pending, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -884,17 +1025,22 @@ __device__ __forceinline__ uint64_t run_dfs1(const KArgs &a, const Sub &s, const
         }
       }
       // Descend into the lowest candidate.
-      c = (uint32_t)__builtin_ctzll(L.v);
-      const uint64_t cbit = 1ull << c;
-      L.v &= ~cbit;
-      L.p &= ~cbit;
-      L.x |= cbit;
-      StP rec = stk + (uint64_t)level * LS;
-      if (fresh) store_lvl1(rec, Wx, L);
-      else store_pxv1(rec, L);
-      par = L;
+      uint32_t jt = 0;
+      if constexpr (NW > 1) jt = L.v[0] ? 0u : 1u;
+      const uint32_t bt = (uint32_t)__builtin_ctzll(pick<NW>(L.v, jt));
+      c = 64u * jt + bt;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        const uint64_t cbit = (uint32_t)k == jt ? 1ull << bt : 0ull;
+        L.v[k] &= ~cbit;
+        L.p[k] &= ~cbit;
+        L.x[k] |= cbit;
+      }
+      uint64_t *rec = stk + (uint64_t)level * LS;
+      if (fresh) store_lvlr<NW, NX>(rec, Wx, L);
+      else store_pxvr<NW, NX>(rec, L);
     }
-    L = child1(par, c, rowP, rowX, gh, acc);
+    L = child_r<NW, NX>(L, c, rw, acc);
     fresh = true;
     ++level;
   }
@@ -1211,8 +1357,10 @@ __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, ui
   s.Plist = P;
   s.bm = bm_g;
   s.gP = gp_g;
-  // |P0| <= 64 and |X0'| <= 64: the single-word path (rows in registers).
-  const bool one_word = a.word_path && W == 1 && Wx <= 1;
+  // |P0| <= 64 or <= 128 and |X0'| <= 64: the register paths (rows in
+  // registers).
+  const bool reg1 = a.word_path && W == 1 && Wx <= 1;
+  const bool reg2 = a.word_path && W == 2 && Wx <= 1;
   // Level 0: R = {}, P = P0, XP = {}, XX = X0'.
   Lvl L0;
   L0.rk = 0;
@@ -1224,23 +1372,29 @@ __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, ui
   L0.xxcnt = (uint32_t)xq;
   L0.xlo = 0;
   L0.xhi = Wx;
+  if (reg1 || reg2) {
+    // The stack in LDS when it fits beside the bitmap, as for run_dfs.
+    const bool sl = in_lds && lds_words(s) * 8 <= a.lds_bytes;
+    uint64_t *stk = sl ? (uint64_t *)(lds + nbw + p) : stk_g;
+    for (uint32_t j = l; j < Wx; j += 64) stk[4 * W + j] = low_mask(xq - (uint64_t)j * 64);
+    store_rpx(stk, s, L0);
+    wave_fence();
+    const uint64_t *bmp = in_lds ? (const uint64_t *)lds : bm_g, *gpp = in_lds ? (const uint64_t *)(lds + nbw) : gp_g;
+    if (reg1)
+      return run_dfsr<1, 1>(a, s, bmp, gpp, stk, sl, stk_g, acc, s_hist, me, published, eager, in_lds);
+    return run_dfsr<2, 1>(a, s, bmp, gpp, stk, sl, stk_g, acc, s_hist, me, published, eager, in_lds);
+  }
   if (in_lds && lds_words(s) * 8 <= a.lds_bytes) {
     lds64 *st = lds + nbw + p;
     for (uint32_t j = l; j < Wx; j += 64) st[4 * W + j] = low_mask(xq - (uint64_t)j * 64);
     store_rpx(st, s, L0);
     wave_fence();
-    if (one_word)
-      return run_dfs1(a, s, (const uint64_t *)lds, (const uint64_t *)(lds + nbw), st, stk_g, acc, s_hist, me,
-                      published, eager, true);
     return run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + nbw), st, stk_g, acc, s_hist, me, published,
                    eager, true);
   }
   for (uint32_t j = l; j < Wx; j += 64) stk_g[4 * W + j] = low_mask(xq - (uint64_t)j * 64);
   store_rpx(stk_g, s, L0);
   wave_fence();
-  if (one_word)
-    return run_dfs1(a, s, in_lds ? (const uint64_t *)lds : bm_g, in_lds ? (const uint64_t *)(lds + nbw) : gp_g,
-                    stk_g, stk_g, acc, s_hist, me, published, eager, in_lds);
   if (in_lds)  // bitmap and label hashes in LDS, the stack (too deep for LDS) in HBM
     return run_dfs(a, s, (const lds64 *)lds, (const lds64 *)(lds + nbw), stk_g, stk_g, acc, s_hist, me, published,
                    eager, true);

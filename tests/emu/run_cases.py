@@ -84,6 +84,14 @@ def cases():
     yield "word path sharing", lambda: same(R.solve(sg, waves_per_cu=wpc, share_bound=1, want_root_stats=True), sref)
     yield "bitmap tier sharing", lambda: same(R.solve(sg, waves_per_cu=wpc, share_bound=1, word_path=False,
                                                       want_root_stats=True), sref)
+    # vertex roots with |P0| in (64, 128] and |X0'| <= 64 (two-word register
+    # path) beside single-word ones
+    tg = graph(random_graph(300, 0.3, 32))
+    tref = oracle_for(tg, root_mode=1)
+    for tau in (-1, 2):
+        yield (f"two-word vertex roots tau={tau}",
+               lambda tau=tau: same(R.solve(tg, waves_per_cu=wpc, root_mode=R.ROOT_VERTEX, share_bound=tau,
+                                            want_root_stats=True), tref))
     vg = graph(random_graph(80, 0.4, 5))
     yield "vertex roots", lambda: same(R.solve(vg, waves_per_cu=wpc, root_mode=R.ROOT_VERTEX, want_root_stats=True),
                                        oracle_for(vg, root_mode=1))
