@@ -1300,10 +1300,25 @@ __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, ui
     for (uint64_t i = l; i < nbw; i += 64) T[i] = 0;
     for (uint64_t i = l; i < p; i += 64) G[i] = a.label_hash[P[i]];
     wave_fence();
-    for (uint64_t i = l; i < rows; i += 64) {
+    // With fewer than 64 rows, g = 2 or 4 lanes share a row, each matching
+    // one share of its P range (a row's cost is a chain of dependent loads
+    // into adjacency lists; splitting the range shortens the chain).  Bits
+    // are set with atomic ORs, as lanes of one row write the same words.
+    const uint32_t g = rows <= 16 ? 4u : (rows <= 32 ? 2u : 1u);
+    const uint32_t gs = g == 4 ? 2u : (g == 2 ? 1u : 0u);
+    for (uint64_t t = l; t < rows << gs; t += 64) {
+      const uint64_t i = t >> gs;
+      const uint32_t sh = (uint32_t)t & (g - 1);
+      const uint32_t j0 = i < p ? (uint32_t)i + 1 : 0u;  // the row's P range [j0, p)
+      const uint32_t q = ((uint32_t)p - j0 + g - 1) >> gs;
+      const uint32_t jlo = j0 + sh * q;
+      const uint32_t jhi = jlo + q < (uint32_t)p ? jlo + q : (uint32_t)p;
+      if (jlo >= jhi) continue;
       if (i < p) {
         const uint32_t u = P[i];
-        match_run(col, a.outoff[u], a.rowoff[u + 1], P, (uint32_t)i + 1, (uint32_t)p, [&](uint32_t j) {
+        const uint64_t e = a.rowoff[u + 1];
+        const uint64_t rb = sh ? lower_bound(col, a.outoff[u], e, P[jlo]) : a.outoff[u];
+        match_run(col, rb, e, P, jlo, jhi, [&](uint32_t j) {
           or64(&T[(uint64_t)(j >> 6) * rows_pad + i], 1ull << (j & 63));
           or64(&T[(uint64_t)(i >> 6) * rows_pad + j], 1ull << (i & 63));
           return false;
@@ -1311,20 +1326,20 @@ __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, ui
       } else {
         const uint32_t u = X[i - p];
         const uint64_t e = a.rowoff[u + 1];
-        const uint64_t rb = lower_bound(col, a.outoff[u], e, above + 1);
+        const uint64_t rb = lower_bound(col, a.outoff[u], e, sh ? P[jlo] : above + 1);
         uint32_t curw = ~0u;
         uint64_t accw = 0;
-        match_run(col, rb, e, P, 0, (uint32_t)p, [&](uint32_t b) {
+        match_run(col, rb, e, P, jlo, jhi, [&](uint32_t b) {
           const uint32_t w = b >> 6;
           if (w != curw) {
-            if (curw != ~0u) T[(uint64_t)curw * rows_pad + i] = accw;
+            if (curw != ~0u) or64(&T[(uint64_t)curw * rows_pad + i], accw);
             curw = w;
             accw = 0;
           }
           accw |= 1ull << (b & 63);
           return false;
         });
-        if (curw != ~0u) T[(uint64_t)curw * rows_pad + i] = accw;
+        if (curw != ~0u) or64(&T[(uint64_t)curw * rows_pad + i], accw);
       }
     }
     wave_fence();
