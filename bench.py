@@ -6,7 +6,7 @@ the graph, the root list and all scratch already resident in HBM (a
 device-resident plan); the timed region covers exactly `--steps` passes,
 synchronised on both sides.  Prints one JSON line.
 
-Workloads (synthetic, generated on the host):
+Workloads (synthetic, generated on the host by rdmce_amd.workloads):
   rmat20-core499  R-MAT scale 20, edge factor 16, (a,b,c,d) =
                   (0.57,0.19,0.19,0.05), seed 2, symmetrised, deduplicated,
                   no self loops, CSR in degeneracy order.  Enumerated: every

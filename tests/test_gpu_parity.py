@@ -169,6 +169,21 @@ def test_sharing_single_heavy_root():
     assert out.shared_tasks > 0
 
 
+@pytest.mark.parametrize("tau", [-1, 2])
+def test_register_paths_match_bitmap_tiers(tau):
+    """Vertex roots of G(300, 0.3) have 65..81 candidates (two-word register
+    path) or fewer (single-word); the same graph with the register paths off
+    runs every root in the bitmap tiers.  Both equal the oracle, per-root
+    node counts included."""
+    banner(f"register paths vs bitmap tiers tau={tau}")
+    g = R.Graph.from_edges(random_graph(300, 0.3, 32))
+    g.SortByOrder(R.OrderType.DEG)
+    ref = oracle_for(g, root_mode=1, root_nodes=True)
+    for wp in (True, False):
+        out = R.solve(g, root_mode=R.ROOT_VERTEX, share_bound=tau, word_path=wp, want_root_stats=True)
+        check_same(out, ref, nodes=True)
+
+
 def test_rmat20_core499_full_size_golden():
     """The bench workload at full size against the committed CPU-oracle
     golden (tests/golden/make_workload_golden.py): 13.2M roots, ~1.0e9 cliques."""
