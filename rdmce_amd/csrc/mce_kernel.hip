@@ -655,8 +655,8 @@ __device__ __forceinline__ uint64_t run_dfs(const KArgs &a, const Sub &s, BmP bm
 // (no dynamic register indexing).  Level records keep the layout of the
 // other tiers (R P XP V XX key meta, words 0 .. 4 NW + Wx + 2 of LS), so
 // publication, task claims and thieves are unchanged.  A record is written
-// only when the search descends from its level (lane i stores word i;
-// later reads of it are by the same lanes), so leaves write nothing.  Same
+// only when the search descends from its level (by lane 0; other lanes read
+// it after a wave_fence), so leaves write nothing.  Same
 // search tree, node counts and accounting as run_dfs.
 // ---------------------------------------------------------------------------
 template <int NW, int NX>
@@ -686,8 +686,9 @@ __device__ __forceinline__ uint64_t writelane64(uint64_t old, uint64_t v, uint32
   return lane() == ln ? v : old;
 }
 
-// Whole record of a level.  XX words: [xlo, xhi) spans the non-zero ones
-// (both 0 when XX is empty), as make_child writes them.
+// Whole record of a level, written by lane 0 (its words are read back by
+// lanes 0 .. 4 NW + Wx + 2 after a wave_fence).  XX words: [xlo, xhi) spans
+// the non-zero ones (both 0 when XX is empty), as make_child writes them.
 template <int NW, int NX, class Ptr>
 __device__ __forceinline__ void store_lvlr(Ptr rec, uint32_t Wx, const LvlR<NW, NX> &L) {
   uint32_t xc = 0, xlo = 0, xhi = 0;
@@ -698,34 +699,34 @@ __device__ __forceinline__ void store_lvlr(Ptr rec, uint32_t Wx, const LvlR<NW, 
       xlo = (uint32_t)j;
       if (!xhi) xhi = (uint32_t)j + 1;
     }
-  uint64_t w = 0;
+  if (lane() == 0) {
 #pragma unroll
-  for (int k = 0; k < NW; ++k) {
-    w = writelane64(w, L.r[k], k);
-    w = writelane64(w, L.p[k], NW + k);
-    w = writelane64(w, L.x[k], 2 * NW + k);
-    w = writelane64(w, L.v[k], 3 * NW + k);
+    for (int k = 0; k < NW; ++k) {
+      rec[k] = L.r[k];
+      rec[NW + k] = L.p[k];
+      rec[2 * NW + k] = L.x[k];
+      rec[3 * NW + k] = L.v[k];
+    }
+#pragma unroll
+    for (int j = 0; j < NX; ++j)
+      if ((uint32_t)j < Wx) rec[4 * NW + j] = L.xx[j];
+    rec[4 * NW + Wx] = L.key;
+    rec[4 * NW + Wx + 1] = (uint64_t)L.rsize | ((uint64_t)xc << 32);
+    rec[4 * NW + Wx + 2] = (uint64_t)xlo | ((uint64_t)xhi << 32);
   }
-#pragma unroll
-  for (int j = 0; j < NX; ++j)
-    if ((uint32_t)j < Wx) w = writelane64(w, L.xx[j], 4 * NW + j);
-  w = writelane64(w, L.key, 4 * NW + Wx);
-  w = writelane64(w, (uint64_t)L.rsize | ((uint64_t)xc << 32), 4 * NW + Wx + 1);
-  w = writelane64(w, (uint64_t)xlo | ((uint64_t)xhi << 32), 4 * NW + Wx + 2);
-  if (lane() < 4 * NW + Wx + 3) rec[lane()] = w;
 }
 
-// P, XP and V of a level whose record exists (lanes NW .. 4 NW - 1).
+// P, XP and V of a level whose record exists (lane 0).
 template <int NW, int NX, class Ptr>
 __device__ __forceinline__ void store_pxvr(Ptr rec, const LvlR<NW, NX> &L) {
-  uint64_t w = 0;
+  if (lane() == 0) {
 #pragma unroll
-  for (int k = 0; k < NW; ++k) {
-    w = writelane64(w, L.p[k], NW + k);
-    w = writelane64(w, L.x[k], 2 * NW + k);
-    w = writelane64(w, L.v[k], 3 * NW + k);
+    for (int k = 0; k < NW; ++k) {
+      rec[NW + k] = L.p[k];
+      rec[2 * NW + k] = L.x[k];
+      rec[3 * NW + k] = L.v[k];
+    }
   }
-  if (lane() - (uint32_t)NW < 3u * NW) rec[lane()] = w;
 }
 
 template <int NW, int NX, class Ptr>
@@ -886,7 +887,10 @@ __device__ __forceinline__ uint32_t eval_r(const KArgs &a, const Sub &s, LvlR<NW
 // descents) and the serving of a frozen level's tasks follow run_dfs.  The
 // stack is addressed through a flat pointer (LDS or HBM, stk_lds telling
 // which): a node touches it at most once, and one instantiation per width
-// keeps the kernel's code and register allocation within budget.
+// keeps the kernel's code and register allocation within budget.  Lane 0
+// writes the records (no per-lane select of uniform words), and every read
+// of one by other lanes follows a wave_fence; the backtrack that follows a
+// store comes a whole subtree later, so the fence rarely waits.
 template <int NW, int NX>
 __device__ __forceinline__ uint64_t run_dfsr(const KArgs &a, const Sub &s, const uint64_t *bm, const uint64_t *gP,
                                              uint64_t *stk, bool stk_lds, uint64_t *stk_g, Acc &acc, ldsu64 *s_hist,
@@ -951,6 +955,7 @@ __device__ __forceinline__ uint64_t run_dfsr(const KArgs &a, const Sub &s, const
         if (l == 0) atomicAdd(a.owner_tasks, 1ull);
         // candidate idx of the frozen V, with P and XP as sequential DFS
         // holds them when it reaches it (as materialize does)
+        wave_fence();
         L = load_lvlr<NW, NX>(stk + (uint64_t)S * LS, Wx);
         uint32_t before = 0, jt = 0;
         if constexpr (NW > 1) {
@@ -978,6 +983,7 @@ __device__ __forceinline__ uint64_t run_dfsr(const KArgs &a, const Sub &s, const
       }
       if (level == 0) return nodes;
       --level;
+      wave_fence();  // lane 0 wrote the record
       L = load_lvlr<NW, NX>(stk + (uint64_t)level * LS, Wx);
       fresh = false;
       Lstar = Lstar < level ? Lstar : level;
@@ -990,7 +996,8 @@ __device__ __forceinline__ uint64_t run_dfsr(const KArgs &a, const Sub &s, const
         if (bcast32(go)) {
           const uint32_t lp = (uint32_t)(S + 1);
           const uint64_t *src = stk + (uint64_t)lp * LS;
-          const uint64_t sw = l < LS ? src[l] : 0ull;  // lane i wrote word i
+          wave_fence();  // lane 0 wrote the record
+          const uint64_t sw = l < LS ? src[l] : 0ull;
           uint32_t n = 0;
 #pragma unroll
           for (int k = 0; k < NW; ++k) n += (uint32_t)__popcll(readlane64(sw, 3 * NW + k));
