@@ -226,7 +226,8 @@ typedef struct mce_tune_result {
  * `stride`-th dispatched root; 1 = all, the faithful choice: a sparse sample
  * changes the load balance the settings are about) for a coordinate search:
  * waves per CU in {8, 12, 16, 20, 24, 28, 32} with the LDS tier on, then the sharing
- * bound in {16, 24, 48, off} at the best occupancy, then the LDS tier off.
+ * bound in {16, 24, 48, off} at the best occupancy, then waves per CU again
+ * when the bound changed, then the LDS tier off.
  * Checks that every setting gives the same clique count and hash, and keeps
  * the fastest for later mce_plan_run calls. */
 mce_status mce_plan_autotune(mce_plan *p, uint64_t stride, mce_tune_result *out);

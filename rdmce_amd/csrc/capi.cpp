@@ -880,7 +880,8 @@ mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
   HIP_TRY(dalloc(&d_sample, sample.size()));
   if (!sample.empty()) HIP_TRY(hipMemcpy(d_sample, sample.data(), sample.size() * 8, hipMemcpyHostToDevice));
   // Waves per CU first (LDS tier on), then the sharing bound at the best
-  // occupancy, then the LDS tier off at the best pair.
+  // occupancy, then waves per CU again when the bound changed, then the LDS
+  // tier off at the best pair.
   const int wpcs[] = {8, 12, 16, 20, 24, 28, 32};
   const int taus[] = {16, 24, 48, -1};
   const int base_wpc = p->waves_per_cu, base_tau = p->share_bound, base_lds = p->lds_tier;
@@ -930,6 +931,19 @@ mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
     if (!st && ms < best) {
       best = ms;
       best_tau = tau;
+    }
+  }
+  // Occupancy again at the chosen sharing bound: without sharing (or with a
+  // different bound) the best number of resident waves can differ.
+  if (!st && best_tau != base_tau) {
+    const int wpc0 = best_wpc;
+    for (int wpc : wpcs) {
+      if ((uint64_t)p->cus * wpc > p->max_waves || st || wpc == wpc0) continue;
+      st = trial(wpc, best_tau, best_lds, ms);
+      if (!st && ms < best) {
+        best = ms;
+        best_wpc = wpc;
+      }
     }
   }
   if (!st && best_lds == 1) {
