@@ -84,3 +84,27 @@ def test_options_struct_roundtrip():
     o = _lib.MceOptions()
     _lib.lib().mce_options_init(ctypes.byref(o))
     assert o.root_mode == _lib.ROOT_EDGE and o.lds_tier == 1 and o.schedule == 0
+    assert o.word_path == 0  # register paths on by default
+
+
+@pytest.mark.parametrize("cname,mirror", [("mce_options", "MceOptions"), ("mce_result", "MceResult"),
+                                           ("mce_tune_result", "MceTuneResult"),
+                                           ("mce_batch_result", "MceBatchResult")])
+def test_struct_layout_matches_header(tmp_path, cname, mirror):
+    """The ctypes mirrors of the ABI's structs have the C structs' sizes and
+    field offsets (compiled from include/rdmce.h with the host compiler)."""
+    cls = getattr(_lib, mirror)
+    fields = [f for f, _ in cls._fields_ if f != "reserved"]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "rdmce.h"\nint main(void) {\n'
+                   f'  printf("size %zu\\n", sizeof({cname}));\n'
+                   + "".join(f'  printf("{f} %zu\\n", offsetof({cname}, {f}));\n' for f in fields)
+                   + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                           text=True).stdout.splitlines())
+    assert int(out["size"]) == ctypes.sizeof(cls)
+    for f in fields:
+        assert int(out[f]) == getattr(cls, f).offset, f"{cname}.{f} offset"
+
