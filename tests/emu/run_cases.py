@@ -89,7 +89,7 @@ def cases():
     tg = graph(random_graph(300, 0.3, 32))
     tref = oracle_for(tg, root_mode=1)
     for tau in (-1, 2):
-        yield (f"two-word vertex roots tau={tau}",
+        yield (f"two-word vertex-mode roots tau={tau}",
                lambda tau=tau: same(R.solve(tg, waves_per_cu=wpc, root_mode=R.ROOT_VERTEX, share_bound=tau,
                                             want_root_stats=True), tref))
     vg = graph(random_graph(80, 0.4, 5))
