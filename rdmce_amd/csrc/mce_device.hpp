@@ -12,7 +12,7 @@ constexpr uint32_t kMaxW = kMaxP / 64;
 constexpr uint32_t kLdsHist = 64;
 constexpr uint32_t kIdleCopies = 64;  // copies of the idle count, one per 128-byte line
 constexpr uint32_t kIdleStride = 32;  // u32 between copies
-constexpr uint32_t kPollSteps = 64;  // descents (tree nodes) between polls for idle waves (a power of two)
+constexpr uint32_t kPollSteps = 128;  // descents (tree nodes) between polls for idle waves (a power of two)
 
 // Error flags raised by the kernels (bit mask in KArgs::err).
 constexpr uint32_t kErrPTooLarge = 1u;       // a root has |P| > kMaxP
