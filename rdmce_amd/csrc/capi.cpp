@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <new>
@@ -156,6 +157,7 @@ mce_status mce_graph_from_csr(uint64_t n, const uint64_t *rowoff, const uint32_t
   return guarded([&] {
     if (!rowoff || (!col && rowoff[n])) return fail(MCE_ERR_ARG, "null CSR array");
     if (n >= (1ull << 32)) return fail(MCE_ERR_UNSUPPORTED, "more than 2^32-1 vertices");
+    if (rowoff[0] != 0) return fail(MCE_ERR_ARG, "rowoff[0] is not 0");
     for (uint64_t v = 0; v < n; ++v) {
       if (rowoff[v + 1] < rowoff[v]) return fail(MCE_ERR_ARG, "row offsets decrease");
       for (uint64_t i = rowoff[v]; i < rowoff[v + 1]; ++i) {
@@ -164,6 +166,23 @@ mce_status mce_graph_from_csr(uint64_t n, const uint64_t *rowoff, const uint32_t
       }
     }
     if (rowoff[n] % 2) return fail(MCE_ERR_ARG, "odd number of adjacency entries");
+    // Symmetry: the kernels build P from out-lists and X from in-lists, so an
+    // edge listed in one direction only would change the results silently.
+    // Rows are sorted (checked above): a binary search per entry.
+    std::atomic<int64_t> bad{-1};
+    parallel_chunks(n, [&](int, uint64_t b, uint64_t e) {
+      for (uint64_t v = b; v < e && bad.load(std::memory_order_relaxed) < 0; ++v)
+        for (uint64_t i = rowoff[v]; i < rowoff[v + 1]; ++i) {
+          const uint32_t u = col[i];
+          if (!std::binary_search(col + rowoff[u], col + rowoff[u + 1], (uint32_t)v)) {
+            bad.store((int64_t)v, std::memory_order_relaxed);
+            break;
+          }
+        }
+    });
+    if (bad.load() >= 0)
+      return fail(MCE_ERR_ARG, "CSR is not symmetric: row " + std::to_string(bad.load()) +
+                                   " lists a neighbour whose row does not list it");
     mce_status s = new_graph(out);
     if (s) return s;
     HostGraph &g = (*out)->g;

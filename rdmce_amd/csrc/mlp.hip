@@ -80,14 +80,21 @@ __device__ __forceinline__ Params params_at(const float *p) {
 
 // Forward for one sample; keeps activations for backward.
 __device__ __forceinline__ float forward_one(const Params &q, const float *x, float *xn, float *h1, float *h2) {
+  // Fully unrolled: the activation arrays stay in registers, indexed by
+  // constants only.
+#pragma unroll
   for (int i = 0; i < kFeat; ++i) xn[i] = (x[i] - q.mean[i]) * q.istd[i];
+#pragma unroll
   for (int j = 0; j < kH1; ++j) {
     float s = q.b1[j];
+#pragma unroll
     for (int i = 0; i < kFeat; ++i) s = fmaf(xn[i], q.W1[i * kH1 + j], s);
     h1[j] = s > 0.f ? s : 0.f;
   }
+#pragma unroll
   for (int j = 0; j < kH2; ++j) {
     float s = q.b2[j];
+#pragma unroll
     for (int i = 0; i < kH1; ++i) s = fmaf(h1[i], q.W2[i * kH2 + j], s);
     h2[j] = s > 0.f ? s : 0.f;
   }
@@ -136,9 +143,12 @@ __global__ __launch_bounds__(kRows) void backward_partial_kernel(const float *pa
     float d = y - Y[r];
     lossv = d * d;
     dy = 2.f * d * scale;
+#pragma unroll
     for (int j = 0; j < kH2; ++j) d2[j] = h2[j] > 0.f ? dy * q.W3[j] : 0.f;
+#pragma unroll
     for (int i = 0; i < kH1; ++i) {
       float acc = 0.f;
+#pragma unroll
       for (int j = 0; j < kH2; ++j) acc = fmaf(q.W2[i * kH2 + j], d2[j], acc);
       d1[i] = h1[i] > 0.f ? acc : 0.f;
     }

@@ -235,9 +235,15 @@ uint64_t emu_ballot(bool pred) {
   return m;
 }
 
+// v_readlane_b32 takes its lane select from an SGPR (or a constant): one
+// index for the whole wave.  Each lane passes its value and its index; lanes
+// that disagree on the index are a program the hardware would run
+// differently (with the first active lane's index), so they fail here.
 uint32_t emu_readlane(uint32_t v, uint32_t lane) {
   if (lane >= 64) emu_fail("readlane index >= 64");
-  const uint64_t *b = exchange(v);
+  const uint64_t *b = exchange(((uint64_t)lane << 32) | v);
+  for (uint32_t i = 0; i < t_wave->lanes; ++i)
+    if ((uint32_t)(b[i] >> 32) != lane) emu_fail("readlane with a lane index that differs between lanes");
   if (lane >= t_wave->lanes) emu_fail("readlane of a lane the wave does not have");
   return (uint32_t)b[lane];
 }

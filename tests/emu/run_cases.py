@@ -92,6 +92,27 @@ def cases():
         yield (f"two-word vertex-mode roots tau={tau}",
                lambda tau=tau: same(R.solve(tg, waves_per_cu=wpc, root_mode=R.ROOT_VERTEX, share_bound=tau,
                                             want_root_stats=True), tref))
+    # the same two-word path, cheaply: three of those vertex roots alone (a
+    # custom subset), with and without sharing
+    ro, co = tg.GetRowOffset(), tg.GetColIdx()
+    outdeg = [int(ro[v + 1]) - int(np.searchsorted(co[ro[v]:ro[v + 1]], v, side="right")) - int(ro[v])
+              for v in range(tg.GetNumVertices())]
+    sub = np.array([v for v in range(len(outdeg)) if 65 <= outdeg[v] <= 80][:3], dtype=np.uint64)
+    ogt = OracleGraph.from_csr(ro, co, tg.GetLabels(), tg.GetDegeneracy())
+    sref_sub = ogt.mce_roots(sub, root_mode=1)
+
+    def two_word_subset(tau):
+        o = R.solve(tg, waves_per_cu=wpc, root_mode=R.ROOT_VERTEX, share_bound=tau, root_order=sub,
+                    want_root_stats=True)
+        if len(sub) != 3:
+            return "expected three roots with 65..80 candidates"
+        if (o.count, o.hash, o.hist_dict()) != (sref_sub.count, sref_sub.hash, sref_sub.hist):
+            return "count/hash/hist differ on the two-word subset"
+        want = np.zeros_like(tref.root_nodes)
+        want[sub.astype(np.int64)] = tref.root_nodes[sub.astype(np.int64)]
+        return None if np.array_equal(o.root_nodes, want) else "per-root node counts differ"
+    for tau in (-1, 2):
+        yield f"two-word subset tau={tau}", lambda tau=tau: two_word_subset(tau)
     vg = graph(random_graph(80, 0.4, 5))
     yield "vertex roots", lambda: same(R.solve(vg, waves_per_cu=wpc, root_mode=R.ROOT_VERTEX, want_root_stats=True),
                                        oracle_for(vg, root_mode=1))

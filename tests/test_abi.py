@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import rdmce_amd as R
@@ -108,3 +109,45 @@ def test_struct_layout_matches_header(tmp_path, cname, mirror):
     for f in fields:
         assert int(out[f]) == getattr(cls, f).offset, f"{cname}.{f} offset"
 
+
+
+def _csr_status(rowoff, col):
+    L = _lib.lib()
+    ro = np.ascontiguousarray(rowoff, dtype=np.uint64)
+    co = np.ascontiguousarray(col, dtype=np.uint32)
+    out = ctypes.c_void_p()
+    st = L.mce_graph_from_csr(ro.size - 1, ro.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                              co.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), None, 0, ctypes.byref(out))
+    if st == 0:
+        L.mce_graph_free(out)
+    return st, L.mce_last_error()
+
+
+def test_graph_from_csr_rejects_asymmetric_and_offset_errors():
+    """mce_graph_from_csr (the input of mce_solve_csr) accepts a symmetric
+    CSR and rejects one that lists an edge in one direction only, a first
+    offset other than 0, self loops and unsorted rows: the kernels build P
+    from out-lists and X from in-lists, so such input would change results
+    without an error."""
+    # path 0-1-2, symmetric
+    assert _csr_status([0, 1, 3, 4], [1, 0, 2, 1])[0] == _lib.MCE_OK
+    # 0 lists 1 and 2, but 2 does not list 0 (even entry count, sorted rows)
+    st, msg = _csr_status([0, 2, 3, 4], [1, 2, 0, 1])
+    assert st == _lib.MCE_ERR_ARG and b"symmetric" in msg
+    # same adjacency behind a non-zero first offset
+    st, msg = _csr_status([2, 3, 5, 6], [9, 9, 1, 0, 2, 1])
+    assert st == _lib.MCE_ERR_ARG and b"rowoff[0]" in msg
+    st, msg = _csr_status([0, 1, 2], [0, 1])
+    assert st == _lib.MCE_ERR_ARG and b"self loop" in msg
+    st, msg = _csr_status([0, 2, 3, 4], [2, 1, 0, 0])
+    assert st == _lib.MCE_ERR_ARG and b"ascending" in msg
+
+
+def test_graph_from_csr_roundtrip_of_a_loaded_graph():
+    g = R.Graph()
+    g.LoadFromFile(os.path.join(ROOT, "tests", "golden", "zachary.txt"))
+    g.SortByOrder(R.OrderType.DEG)
+    h = R.Graph.from_csr(g.GetRowOffset(), g.GetColIdx(), g.GetLabels(), g.GetDegeneracy())
+    assert np.array_equal(h.GetRowOffset(), g.GetRowOffset())
+    assert np.array_equal(h.GetColIdx(), g.GetColIdx())
+    assert h.GetNumEdges() == 78

@@ -15,7 +15,7 @@ LIB = os.path.join(EMU, "_build", "librdmce_emu.so")
 
 QUICK = ["zachary", "closed K5", "closed moon_moser4", "closed cocktail8", "closed multipartite_1_5_9_2",
          "G(60,0.5) tau=1 lds=1", "G(60,0.5) tau=-1 lds=0", "G(120,0.3) tau=4 lds=0", "G(200,0.08) tau=24 lds=1",
-         "vertex roots", "deferred pass", "batch", "plan graph replay"]
+         "vertex roots", "two-word subset", "deferred pass", "batch", "plan graph replay"]
 
 
 @pytest.fixture(scope="module")
