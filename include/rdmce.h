@@ -128,8 +128,14 @@ typedef struct mce_options {
                                 top level at once (a pre-launch split) */
   int32_t word_path;         /* roots with |P0|, |X0'| <= 64 or <= 128 search with their bitmap rows in
                                 registers: 0 = on (default), -1 = off (every root in the bitmap tiers) */
-  int32_t pad0;
-  uint64_t reserved[5];
+  int32_t pack;              /* packed search (no reference counterpart: the reference runs one root per
+                                warp, BkPivotBitBalance.cuh:429-470): 1 = roots with |P0|, |X0'| <= 64, then
+                                <= 128, are searched many per wavefront, one tree node per lane, and only the
+                                wider ones by the one-root-per-wavefront kernel; 0 = off (default, the plain
+                                path).  Never changes a result. */
+  uint32_t pack_slots;       /* subproblems resident per wavefront in the packed search (1..8; 0 = 4) */
+  uint32_t pack_waves_per_cu;/* wavefronts per CU of the packed search (1..32; 0 = 16) */
+  uint64_t reserved[4];
 } mce_options;
 
 void mce_options_init(mce_options *opt);

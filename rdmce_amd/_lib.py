@@ -46,8 +46,10 @@ class MceOptions(Structure):
         ("lds_bytes", c_uint32),
         ("eager", POINTER(ctypes.c_uint8)),
         ("word_path", c_int32),
-        ("pad0", c_int32),
-        ("reserved", c_uint64 * 5),
+        ("pack", c_int32),
+        ("pack_slots", c_uint32),
+        ("pack_waves_per_cu", c_uint32),
+        ("reserved", c_uint64 * 4),
     ]
 
 

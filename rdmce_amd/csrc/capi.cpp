@@ -415,6 +415,18 @@ struct mce_plan {
   unsigned long long *d_gcount = nullptr;
   unsigned long long *d_ghash = nullptr;
   unsigned long long *d_ghist = nullptr;
+  // packed search (pack_kernel.hip): settings and arenas, allocated at its
+  // first run
+  uint64_t max_out = 0;              // largest out-degree: bounds |P0| and the search depth
+  int pack = 0;
+  uint32_t pack_slots = 4;
+  int pack_wpc = 16;
+  uint32_t pack_waves_alloc = 0;     // waves the arenas below are sized for
+  uint64_t pk_stack_words = 0;       // record stack words per wave
+  uint64_t *d_pk_stack = nullptr;
+  uint32_t *d_pk_plist = nullptr;
+  uint64_t *d_wide[2] = {nullptr, nullptr};
+  uint64_t wide_cap = 0;
   // captured run (hipGraph)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -426,7 +438,7 @@ struct mce_batch {
 
 namespace {
 enum Ctr { kCount, kHash, kNodes, kNext, kListCursor, kNDeferred, kDeferNeed, kDeferStackNeed, kShared, kActive,
-           kIdle, kErr, kWords, kBuild, kPublished, kOwnerTasks, kNumCtr };
+           kIdle, kErr, kWords, kBuild, kPublished, kOwnerTasks, kNext2, kNext3, kNWide1, kNWide2, kNumCtr };
 
 template <class T>
 hipError_t dalloc(T **p, uint64_t count) {
@@ -449,7 +461,7 @@ void plan_release(mce_plan *p) {
                   p->d_root_nodes, p->d_root_cycles, p->d_list,     p->d_deferred,    p->d_deferred2,
                   p->d_slots2,   p->d_stacks,      p->d_stacks2,    p->d_recs,        p->d_users,
                   p->d_has_work, p->d_root_graph,  p->d_gcount,     p->d_ghash,       p->d_ghist,
-                  p->d_idle};
+                  p->d_idle,     p->d_pk_stack,    p->d_pk_plist,   p->d_wide[0],     p->d_wide[1]};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &e : p->ev)
@@ -531,6 +543,10 @@ mce_status plan_create_impl(const PlanSpec &spec, const mce_options &opt, mce_pl
   std::vector<uint64_t> roots;
   uint64_t max_out = 0;
   build_roots(g, mode, roots, max_out);
+  p->max_out = max_out;
+  p->pack = opt.pack > 0 ? 1 : 0;
+  p->pack_slots = opt.pack_slots ? std::min<uint32_t>(std::max<uint32_t>(opt.pack_slots, 1), kPackMaxSlots) : 4;
+  p->pack_wpc = opt.pack_waves_per_cu ? (int)std::min<uint32_t>(std::max<uint32_t>(opt.pack_waves_per_cu, 1), 32) : 16;
   p->n = g.n;
   p->n_roots = roots.size();
   p->n_dispatch = p->n_roots;
@@ -696,8 +712,44 @@ KArgs make_args(const mce_plan *p) {
   return a;
 }
 
+// Packed search: records per wave and the depth above which only the top
+// record is expanded, for variant (nw, nx).  A record's depth is at most
+// |R| <= |P0| <= min(64 nw, largest out-degree); expanding only the top record
+// adds at most 64 records per depth level, so pk_thr + 64 (depth + 3) records
+// always suffice.
+constexpr uint64_t kPackThr = 256;
+uint64_t pack_need_records(const mce_plan *p, uint32_t nw) {
+  return kPackThr + 64 * (std::min<uint64_t>(64ull * nw, p->max_out) + 3);
+}
+
+// Allocates the packed search's arenas (once per plan, for 32 waves per CU).
+mce_status ensure_pack(mce_plan *p) {
+  if (p->d_pk_stack && (!p->opt.want_list || p->d_pk_plist)) return MCE_OK;
+  const uint32_t waves = (uint32_t)std::min<uint64_t>(p->max_waves, (uint64_t)p->cus * 32);
+  const uint64_t words = std::max(pack_need_records(p, 1) * pack_rec_words(1, 1),
+                                  pack_need_records(p, 2) * pack_rec_words(2, 2));
+  if (!p->d_pk_stack) {
+    HIP_TRY(dalloc(&p->d_pk_stack, words * waves));
+    p->pk_stack_words = words;
+    p->pack_waves_alloc = waves;
+    p->wide_cap = std::max<uint64_t>(p->n_dispatch, 1);
+    HIP_TRY(dalloc(&p->d_wide[0], p->wide_cap));
+    HIP_TRY(dalloc(&p->d_wide[1], p->wide_cap));
+  }
+  if (p->opt.want_list && !p->d_pk_plist) HIP_TRY(dalloc(&p->d_pk_plist, (uint64_t)waves * kPackMaxSlots * kPackMaxP));
+  return MCE_OK;
+}
+
+uint32_t pack_waves(const mce_plan *p) {
+  return (uint32_t)std::min<uint64_t>(p->pack_waves_alloc, std::max<uint64_t>(1, (uint64_t)p->cus * p->pack_wpc));
+}
+
 // Resets every counter, result and sharing word of a run with fill kernels
-// (graph-capturable) and launches the enumeration kernel.
+// (graph-capturable) and launches the enumeration kernels: with the packed
+// search on, pack<1,1> over the dispatch list, pack<2,2> over the roots it
+// found too wide, and the one-root-per-wave kernel over the rest (each
+// later launch reads its list length from the counter the earlier one
+// filled, so the sequence needs no host round trip).
 hipError_t enqueue_run(const mce_plan *p, const KArgs &a, hipStream_t s) {
   hipError_t e = launch_fill_zero(p->d_ctr, kNumCtr, s);
   if (e == hipSuccess) e = launch_fill_zero(p->d_hist, p->hist_len, s);
@@ -710,8 +762,43 @@ hipError_t enqueue_run(const mce_plan *p, const KArgs &a, hipStream_t s) {
   if (e == hipSuccess && p->d_gcount) e = launch_fill_zero(p->d_gcount, p->n_graphs, s);
   if (e == hipSuccess && p->d_ghash) e = launch_fill_zero(p->d_ghash, p->n_graphs, s);
   if (e == hipSuccess && p->d_ghist) e = launch_fill_zero(p->d_ghist, p->n_graphs * p->g_hist_stride, s);
-  if (e == hipSuccess) e = launch_mce(a, p->waves, (uint32_t)p->waves_per_cu, s);
-  return e;
+  if (e != hipSuccess) return e;
+  if (!p->pack) return launch_mce(a, p->waves, (uint32_t)p->waves_per_cu, s);
+  const uint32_t pw = pack_waves(p);
+  for (int v = 0; v < 2 && e == hipSuccess; ++v) {
+    const uint32_t nw = v == 0 ? 1 : 2, nx = nw;
+    KArgs b = a;
+    b.pk_stack = p->d_pk_stack;
+    b.pk_cap = p->pk_stack_words / pack_rec_words(nw, nx);
+    b.pk_thr = (uint32_t)(b.pk_cap - 64 * (std::min<uint64_t>(64ull * nw, p->max_out) + 3));
+#ifdef RDMCE_PACK_THR_MAX  // test builds: reach the depth-first expansion on small graphs
+    b.pk_thr = std::min<uint32_t>(b.pk_thr, RDMCE_PACK_THR_MAX);
+#endif
+    // the two-word variant's slots are 3.5x larger: half as many, and as
+    // many as one workgroup's LDS holds
+    uint32_t slots = v == 0 ? p->pack_slots : std::max<uint32_t>(1, p->pack_slots / 2);
+    while (slots > 1 && pack_lds_bytes(nw, nx, slots, a.root_graph != nullptr) > p->lds_per_block_max) --slots;
+    b.pk_slots = slots;
+    b.pk_plist = p->d_pk_plist;
+    b.wide = p->d_wide[v];
+    b.n_wide = p->d_ctr + (v == 0 ? kNWide1 : kNWide2);
+    b.eager = nullptr;
+    if (v == 1) {
+      b.perm = p->d_wide[0];
+      b.reverse = 0;
+      b.n_dispatch_dev = p->d_ctr + kNWide1;
+      b.next_root = p->d_ctr + kNext2;
+    }
+    e = launch_pack(b, v, pw, s);
+  }
+  if (e != hipSuccess) return e;
+  KArgs c = a;
+  c.perm = p->d_wide[1];
+  c.reverse = 0;
+  c.eager = nullptr;
+  c.n_dispatch_dev = p->d_ctr + kNWide2;
+  c.next_root = p->d_ctr + kNext3;
+  return launch_mce(c, p->waves, (uint32_t)p->waves_per_cu, s);
 }
 
 struct RunOut {
@@ -739,6 +826,10 @@ mce_status run_core(mce_plan *p, const KArgs &a0, bool allow_graph, RunOut &ro) 
     HIP_TRY(hipStreamSynchronize(s));
     std::memset(ro.ctr, 0, sizeof(ro.ctr));
     return MCE_OK;
+  }
+  if (p->pack) {
+    mce_status st = ensure_pack(p);
+    if (st) return st;
   }
   const bool use_graph = allow_graph && p->opt.use_graph;
   if (use_graph && !p->graph_exec) {
@@ -830,6 +921,7 @@ mce_status run_core(mce_plan *p, const KArgs &a0, bool allow_graph, RunOut &ro) 
   if (err & kErrHistOverflow) return fail(MCE_ERR_INTERNAL, "clique size beyond histogram");
   if (err & kErrListFull) return fail(MCE_ERR_NOMEM, "clique list capacity exceeded");
   if (err & kErrDeferFull) return fail(MCE_ERR_NOMEM, "deferred-root list overflow");
+  if (err & kErrPackStack) return fail(MCE_ERR_INTERNAL, "packed search: record stack full");
   return MCE_OK;
 }
 

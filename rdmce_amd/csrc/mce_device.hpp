@@ -19,6 +19,7 @@ constexpr uint32_t kErrPTooLarge = 1u;       // a root has |P| > kMaxP
 constexpr uint32_t kErrHistOverflow = 2u;    // clique larger than hist_len - 1
 constexpr uint32_t kErrDeferFull = 4u;       // deferred-root list full
 constexpr uint32_t kErrListFull = 8u;        // clique list capacity exceeded
+constexpr uint32_t kErrPackStack = 16u;      // packed search: record stack full
 
 // One root's subproblem, as every wavefront working on it sees it.  The
 // pointers are HBM copies (the owner may work on an LDS copy of bm and gP).
@@ -113,11 +114,46 @@ struct KArgs {
   unsigned long long *g_hash;
   unsigned long long *g_hist;
   uint32_t g_hist_stride;
+  // Dispatch count read by the kernel at launch instead of n_dispatch
+  // (nullable): the length of a root list that an earlier launch of the same
+  // run appended to.
+  const unsigned long long *n_dispatch_dev;
+  // Packed search (pack_kernel.hip): per-wave record stack of pk_cap records,
+  // pk_slots subproblems resident per wave in LDS, and the roots too wide for
+  // the launched variant, appended in the order met.
+  uint64_t *pk_stack;
+  uint64_t pk_cap;
+  uint32_t pk_slots;
+  uint32_t pk_thr;                  // stack depth above which only the top record is expanded
+  uint32_t *pk_plist;               // per wave pk_slots * kPackMaxP P0 vertex ids (clique lists)
+  uint64_t *wide;
+  unsigned long long *n_wide;
 };
+
+// Packed search variants: |P0| <= 64 NW and |X0'| <= 64 NX.
+constexpr uint32_t kPackMaxP = 128;
+constexpr uint32_t kPackMaxSlots = 8;
+// Record words of variant (nw, nx): R P XP V (nw each), XX (nx), key, meta,
+// remaining candidates (nw), rounded up to an even count.
+constexpr uint32_t pack_rec_words(uint32_t nw, uint32_t nx) { return (5 * nw + nx + 2 + 1) / 2 * 2; }
+// LDS words of one resident subproblem: rows of P0 and X0' over P0 (nw words
+// each), columns of the X0' block per P0 vertex (nx words), label hashes.
+constexpr uint32_t pack_slot_words(uint32_t nw, uint32_t nx) { return 64 * (nw * nw + 2 * nw * nx + nw); }
+// Clique sizes below this bound are counted per resident subproblem (batches).
+constexpr uint32_t pack_hist_words(uint32_t nw) { return 64 * nw + 3; }
+// Per-slot bookkeeping words (canon, key, p|xq, base|graph, v0|v1, nodes,
+// refs, count, hash).
+constexpr uint32_t kPackMetaWords = 9;
+// LDS bytes of a wave of the packed search.
+constexpr uint64_t pack_lds_bytes(uint32_t nw, uint32_t nx, uint32_t slots, bool batch) {
+  return 8ull * (kLdsHist + 32 + slots * (kPackMetaWords + pack_slot_words(nw, nx) + (batch ? pack_hist_words(nw) : 0)));
+}
 
 // waves_per_cu selects the kernel variant compiled for that occupancy
 // (<= 16: 4 waves per SIMD, <= 20: 5, <= 24: 6, else 8).
 hipError_t launch_mce(const KArgs &a, uint32_t waves, uint32_t waves_per_cu, hipStream_t stream);
+// Packed search: variant 0 = (NW, NX) = (1, 1), 1 = (2, 2).
+hipError_t launch_pack(const KArgs &a, int variant, uint32_t waves, hipStream_t stream);
 hipError_t launch_copy_probe(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
 // Zero-fills n 64-bit words with a kernel (safe inside captured hipGraphs,
 // unlike memset nodes).

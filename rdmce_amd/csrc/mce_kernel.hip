@@ -27,192 +27,13 @@
 //    BkPivotBitBalance.cuh:480-487).
 #include <type_traits>
 
-#include "mce_device.hpp"
-#include "rng.hpp"
+#include "mce_common.hpp"
 
 // Dynamic LDS of a wave (the launch gives its size).
 extern __shared__ uint64_t lds_dyn[];
 
 namespace rdmce {
 namespace {
-
-__device__ __forceinline__ uint32_t lane() { return __lane_id(); }
-
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t l) {
-  uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
-  uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-// Wave-wide reductions with DPP (row_shr 1/2/4/8 inside each 16-lane row,
-// then row_bcast 15/31 across rows); the result is read from lane 63.
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-  const uint32_t id = 0xFFFFFFFFu;
-  uint32_t t;
-  t = __builtin_amdgcn_update_dpp(id, v, 0x111, 0xF, 0xF, false); v = v < t ? v : t;
-  t = __builtin_amdgcn_update_dpp(id, v, 0x112, 0xF, 0xF, false); v = v < t ? v : t;
-  t = __builtin_amdgcn_update_dpp(id, v, 0x114, 0xF, 0xF, false); v = v < t ? v : t;
-  t = __builtin_amdgcn_update_dpp(id, v, 0x118, 0xF, 0xF, false); v = v < t ? v : t;
-  t = __builtin_amdgcn_update_dpp(id, v, 0x142, 0xA, 0xF, false); v = v < t ? v : t;
-  t = __builtin_amdgcn_update_dpp(id, v, 0x143, 0xC, 0xF, false); v = v < t ? v : t;
-  return __builtin_amdgcn_readlane(v, 63);
-}
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-  uint32_t t;
-  t = __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, false); v = v > t ? v : t;
-  t = __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, false); v = v > t ? v : t;
-  t = __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, false); v = v > t ? v : t;
-  t = __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, false); v = v > t ? v : t;
-  t = __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, false); v = v > t ? v : t;
-  t = __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xC, 0xF, false); v = v > t ? v : t;
-  return __builtin_amdgcn_readlane(v, 63);
-}
-// Population count of a set whose words are held one per lane (lanes < W).
-__device__ __forceinline__ uint32_t set_popc(uint64_t wk, uint32_t W) {
-  uint32_t s = 0;
-  for (uint32_t k = 0; k < W; ++k) s += (uint32_t)__popcll(readlane64(wk, k));
-  return s;
-}
-// Number of set bits of `mask` below this lane.
-__device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-// Orders this wave's earlier global stores before its later loads.
-__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
-
-__device__ __forceinline__ uint64_t lower_bound(const uint32_t *a, uint64_t lo, uint64_t hi, uint32_t key) {
-  while (lo < hi) {
-    uint64_t mid = (lo + hi) >> 1;
-    if (a[mid] < key) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ uint64_t low_mask(uint64_t bits) {  // bits in [0, 64]
-  return bits >= 64 ? ~0ull : ((1ull << bits) - 1);
-}
-__device__ __forceinline__ uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
-
-// Calls on_hit(b) for every index b (ascending) of the sorted list P[0..p)
-// with P[b] in N(u); stops early when on_hit returns true.  Binary search of
-// P into N(u) when N(u) is much longer, else a linear merge.
-template <class F>
-__device__ __forceinline__ void for_each_p_in_nu(const KArgs &a, uint32_t u, const uint32_t *P, uint32_t p, F on_hit) {
-  const uint32_t *nu = a.col + a.rowoff[u];
-  uint64_t du = a.rowoff[u + 1] - a.rowoff[u];
-  uint64_t pos = lower_bound(nu, 0, du, P[0]);
-  if ((uint64_t)p * 16 < du - pos) {
-    for (uint32_t b = 0; b < p && pos < du; ++b) {
-      uint32_t key = P[b];
-      pos = lower_bound(nu, pos, du, key);
-      if (pos < du && nu[pos] == key && on_hit(b)) return;
-    }
-  } else {
-    uint32_t b = 0;
-    while (pos < du && b < p) {
-      uint32_t x = nu[pos], y = P[b];
-      if (x < y) ++pos;
-      else if (x > y) ++b;
-      else {
-        if (on_hit(b)) return;
-        ++pos;
-        ++b;
-      }
-    }
-  }
-}
-
-// Per-wave accumulators (lane 0 flushes them at exit).  `words` counts the
-// 64-bit bitmap words the search reads; `build` the bytes the subproblem
-// build moves (both adjacency lists, the candidate list, the bitmap).
-// Calls on_hit(j) for every j in [j0, p), ascending, with P[j] in the sorted
-// run col[b, e); stops early when on_hit returns true.  Binary search of the
-// shorter side into the longer when their lengths differ by 8x or more, a
-// linear merge otherwise.
-template <class F>
-__device__ __forceinline__ void match_run(const uint32_t *col, uint64_t b, uint64_t e, const uint32_t *P, uint32_t j0,
-                                          uint32_t p, F on_hit) {
-  if (b >= e || j0 >= p) return;
-  const uint64_t n = e - b;
-  const uint32_t m = p - j0;
-  if ((uint64_t)m * 8 < n) {
-    uint64_t pos = b;
-    for (uint32_t j = j0; j < p && pos < e; ++j) {
-      const uint32_t key = P[j];
-      pos = lower_bound(col, pos, e, key);
-      if (pos < e && col[pos] == key && on_hit(j)) return;
-    }
-  } else if (n * 8 < (uint64_t)m) {
-    uint32_t lo = j0;
-    for (uint64_t i = b; i < e && lo < p; ++i) {
-      const uint32_t w = col[i];
-      uint32_t x = lo, z = p;
-      while (x < z) {
-        uint32_t mid = (x + z) >> 1;
-        if (P[mid] < w) x = mid + 1;
-        else z = mid;
-      }
-      lo = x;
-      if (x < p && P[x] == w) {
-        if (on_hit(x)) return;
-        lo = x + 1;
-      }
-    }
-  } else {
-    uint64_t i = b;
-    uint32_t j = j0;
-    while (i < e && j < p) {
-      const uint32_t x = col[i], y = P[j];
-      if (x < y) ++i;
-      else if (x > y) ++j;
-      else {
-        if (on_hit(j)) return;
-        ++i;
-        ++j;
-      }
-    }
-  }
-}
-
-// Relaxed 64-bit OR into the bitmap (HBM or LDS).
-__device__ __forceinline__ void or64(uint64_t *p, uint64_t v) {
-  __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void or64(__attribute__((address_space(3))) uint64_t *p, uint64_t v) {
-  __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-struct Acc {
-  unsigned long long count;
-  unsigned long long hash;
-  unsigned long long words;
-  unsigned long long build;
-};
-
-__device__ __forceinline__ unsigned int ld_relaxed(unsigned int *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long ld_relaxed(unsigned long long *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned int ld_acquire(unsigned int *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Sleeps about 256 * n cycles (s_sleep takes an immediate operand).
-__device__ __forceinline__ void nap(uint32_t n) {
-  for (uint32_t i = 0; i < n; ++i) __builtin_amdgcn_s_sleep(4);
-}
-__device__ __forceinline__ uint32_t bcast32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ uint64_t bcast64(uint64_t v) {
-  return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
-         __builtin_amdgcn_readfirstlane((uint32_t)v);
-}
-
-// LDS pointers (address space 3): the LDS tiers and the per-wave clique-size
-// histogram.
-using lds64 = __attribute__((address_space(3))) uint64_t;
-using ldsu64 = __attribute__((address_space(3))) unsigned long long;
 
 __device__ __forceinline__ void emit(const KArgs &a, const Sub &s, Acc &acc, ldsu64 *s_hist,
                                      uint32_t size, uint64_t key, uint64_t rk) {
@@ -850,7 +671,7 @@ __device__ __forceinline__ uint32_t eval_r(const KArgs &a, const Sub &s, LvlR<NW
     }
     return 0;
   }
-  acc.words += (cw + xw + 1) * NW;
+  acc.words += (cw + xw) * NW;
   uint32_t best = ~0u;
 #pragma unroll
   for (int j = 0; j < NW; ++j) {
@@ -870,6 +691,7 @@ __device__ __forceinline__ uint32_t eval_r(const KArgs &a, const Sub &s, LvlR<NW
   }
   const uint32_t k = wave_min_u32(best);
   if ((k >> 16) == 0) return 0;  // an excluded row covers P: no maximal clique below
+  acc.words += NW;               // the pivot's row
   const uint32_t piv = k & 0xFFFFu;
   const bool in_p = piv < p;
   const uint32_t row = in_p ? piv : piv - p, js = row >> 6, ls = row & 63;
@@ -1156,8 +978,6 @@ __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, ui
                                uint64_t canon, Acc &acc, ldsu64 *s_hist, bool &published,
                                bool eager) {
   const uint32_t l = lane();
-  const uint64_t desc = a.roots[canon];
-  const uint32_t *col = a.col;
   uint32_t *Pl = (uint32_t *)slot;  // P list, then X (later X') list
   const uint64_t ccap = a.slot_bytes / 4;
   Sub s;
@@ -1165,81 +985,8 @@ __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, ui
   s.bm_owner = me;
   s.graph = a.root_graph ? a.root_graph[canon] : 0;
   uint64_t p = 0, x = 0;
-  uint32_t above;  // every P vertex lies above this vertex, every X vertex below it
-  if (a.root_mode == 2) {
-    s.v0 = (uint32_t)(desc >> 32);
-    s.v1 = (uint32_t)desc;
-    s.base_size = 2;
-    s.base_key = a.label_hash[s.v0] + a.label_hash[s.v1];
-    above = s.v1;
-    const uint64_t b0 = a.rowoff[s.v0], e0 = a.rowoff[s.v0 + 1], o0 = a.outoff[s.v0];
-    const uint64_t b1 = a.rowoff[s.v1], e1 = a.rowoff[s.v1 + 1], o1 = a.outoff[s.v1];
-    // P = N+(v1) & (N+(v0) above v1): iterate the shorter run, binary-search
-    // the longer (BkPivotBitBalance.cuh:449-469 builds it from N(v0) & N(v1)).
-    {
-      const uint64_t s0 = lower_bound(col, o0, e0, s.v1 + 1);
-      uint64_t ab = o1, ae = e1, bb = s0, be = e0;
-      if (ae - ab > be - bb) {
-        uint64_t t0 = ab, t1 = ae;
-        ab = bb;
-        ae = be;
-        bb = t0;
-        be = t1;
-      }
-      for (uint64_t base = ab; base < ae; base += 64) {
-        const uint64_t i = base + l;
-        uint32_t val = 0;
-        bool found = false;
-        if (i < ae) {
-          val = col[i];
-          const uint64_t pos = lower_bound(col, bb, be, val);
-          found = pos < be && col[pos] == val;
-        }
-        const uint64_t m = __ballot(found);
-        if (found && p + rank_below(m) < ccap) Pl[p + rank_below(m)] = val;
-        p += __popcll(m);
-      }
-      acc.build += 4 * ((e1 - o1) + (e0 - s0));
-    }
-    // X = (N(v0) below v1) & N-(v1); with P empty only its emptiness matters.
-    {
-      uint64_t ab = b0, ae = lower_bound(col, b0, e0, s.v1), bb = b1, be = o1;
-      if (ae - ab > be - bb) {
-        uint64_t t0 = ab, t1 = ae;
-        ab = bb;
-        ae = be;
-        bb = t0;
-        be = t1;
-      }
-      acc.build += 4 * ((ae - ab) + (be - bb));
-      for (uint64_t base = ab; base < ae; base += 64) {
-        const uint64_t i = base + l;
-        uint32_t val = 0;
-        bool found = false;
-        if (i < ae) {
-          val = col[i];
-          const uint64_t pos = lower_bound(col, bb, be, val);
-          found = pos < be && col[pos] == val;
-        }
-        const uint64_t m = __ballot(found);
-        if (found && p + x + rank_below(m) < ccap) Pl[p + x + rank_below(m)] = val;
-        x += __popcll(m);
-        if (p == 0 && x) break;
-      }
-    }
-  } else {
-    s.v0 = (uint32_t)desc;
-    s.v1 = 0;
-    s.base_size = 1;
-    s.base_key = a.label_hash[s.v0];
-    above = s.v0;
-    const uint64_t b = a.rowoff[s.v0], e = a.rowoff[s.v0 + 1], o = a.outoff[s.v0];
-    p = e - o;
-    x = o - b;
-    for (uint64_t i = l; i < p && i < ccap; i += 64) Pl[i] = col[o + i];
-    for (uint64_t i = l; i < x && p + i < ccap; i += 64) Pl[p + i] = col[b + i];
-    acc.build += 4 * (e - b);
-  }
+  // every P vertex lies above `above`, every X vertex below it
+  const uint32_t above = root_lists(a, canon, Pl, ccap, s, p, x, acc);
   if (p == 0) {
     if (x == 0) {
       s.W = 0;
@@ -1263,24 +1010,7 @@ __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, ui
   const uint32_t *P = Pl;
   uint32_t *X = Pl + p;
   // X' = X rows with a neighbour in P, compacted in place (graph_gpu.cuh:90-106).
-  uint64_t xq = 0;
-  for (uint64_t base = 0; base < x; base += 64) {
-    const uint64_t i = base + l;
-    bool keep = false;
-    uint32_t xv = 0;
-    if (i < x) {
-      xv = X[i];
-      const uint64_t e = a.rowoff[xv + 1];
-      const uint64_t rb = lower_bound(col, a.outoff[xv], e, above + 1);
-      match_run(col, rb, e, P, 0, (uint32_t)p, [&](uint32_t) {
-        keep = true;
-        return true;
-      });
-    }
-    const uint64_t m = __ballot(keep);
-    if (keep) X[xq + rank_below(m)] = xv;
-    xq += __popcll(m);
-  }
+  const uint64_t xq = compact_x(a, P, p, X, x, above);
   wave_fence();
   const uint32_t Wx = (uint32_t)((xq + 63) / 64);
   const uint64_t rows = p + xq;
@@ -1307,48 +1037,12 @@ __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, ui
     for (uint64_t i = l; i < nbw; i += 64) T[i] = 0;
     for (uint64_t i = l; i < p; i += 64) G[i] = a.label_hash[P[i]];
     wave_fence();
-    // With fewer than 64 rows, g = 2 or 4 lanes share a row, each matching
-    // one share of its P range (a row's cost is a chain of dependent loads
-    // into adjacency lists; splitting the range shortens the chain).  Bits
-    // are set with atomic ORs, as lanes of one row write the same words.
-    const uint32_t g = rows <= 16 ? 4u : (rows <= 32 ? 2u : 1u);
-    const uint32_t gs = g == 4 ? 2u : (g == 2 ? 1u : 0u);
-    for (uint64_t t = l; t < rows << gs; t += 64) {
-      const uint64_t i = t >> gs;
-      const uint32_t sh = (uint32_t)t & (g - 1);
-      const uint32_t j0 = i < p ? (uint32_t)i + 1 : 0u;  // the row's P range [j0, p)
-      const uint32_t q = ((uint32_t)p - j0 + g - 1) >> gs;
-      const uint32_t jlo = j0 + sh * q;
-      const uint32_t jhi = jlo + q < (uint32_t)p ? jlo + q : (uint32_t)p;
-      if (jlo >= jhi) continue;
-      if (i < p) {
-        const uint32_t u = P[i];
-        const uint64_t e = a.rowoff[u + 1];
-        const uint64_t rb = sh ? lower_bound(col, a.outoff[u], e, P[jlo]) : a.outoff[u];
-        match_run(col, rb, e, P, jlo, jhi, [&](uint32_t j) {
-          or64(&T[(uint64_t)(j >> 6) * rows_pad + i], 1ull << (j & 63));
-          or64(&T[(uint64_t)(i >> 6) * rows_pad + j], 1ull << (i & 63));
-          return false;
-        });
-      } else {
-        const uint32_t u = X[i - p];
-        const uint64_t e = a.rowoff[u + 1];
-        const uint64_t rb = lower_bound(col, a.outoff[u], e, sh ? P[jlo] : above + 1);
-        uint32_t curw = ~0u;
-        uint64_t accw = 0;
-        match_run(col, rb, e, P, jlo, jhi, [&](uint32_t b) {
-          const uint32_t w = b >> 6;
-          if (w != curw) {
-            if (curw != ~0u) or64(&T[(uint64_t)curw * rows_pad + i], accw);
-            curw = w;
-            accw = 0;
-          }
-          accw |= 1ull << (b & 63);
-          return false;
-        });
-        if (curw != ~0u) or64(&T[(uint64_t)curw * rows_pad + i], accw);
-      }
-    }
+    match_rows(a, P, p, X, rows, above,
+               [&](uint64_t i, uint32_t j) {
+                 or64(&T[(uint64_t)(j >> 6) * rows_pad + i], 1ull << (j & 63));
+                 or64(&T[(uint64_t)(i >> 6) * rows_pad + j], 1ull << (i & 63));
+               },
+               [&](uint64_t i, uint32_t w, uint64_t bits) { or64(&T[(uint64_t)w * rows_pad + i], bits); });
     wave_fence();
     for (uint64_t i = l; i < p; i += 64) {
       uint32_t d = 0;
@@ -1526,6 +1220,7 @@ __device__ __forceinline__ void mce_main(const KArgs &a) {
   // work when no roots are left and no wave is working: only working waves
   // publish tasks, and a thief becomes working before the publisher it
   // took from can finish.
+  const uint64_t n_dispatch = a.n_dispatch_dev ? ld_relaxed((unsigned long long *)a.n_dispatch_dev) : a.n_dispatch;
   bool roots_left = true, is_idle = false;
   bool published = false;  // others may be reading this wave's data region
   uint32_t backoff = 1;
@@ -1546,13 +1241,13 @@ __device__ __forceinline__ void mce_main(const KArgs &a) {
         unsigned long long i = 0;
         if (l == 0) i = atomicAdd(a.next_root, 1ull);
         i = bcast64(i);
-        if (i < a.n_dispatch) {
+        if (i < n_dispatch) {
           if (is_idle) {
             idle_add(a, -1);
             if (l == 0) atomicAdd(a.active, 1u);
             is_idle = false;
           }
-          uint64_t canon = a.perm ? a.perm[i] : (a.reverse ? a.n_dispatch - 1 - i : i);
+          uint64_t canon = a.perm ? a.perm[i] : (a.reverse ? n_dispatch - 1 - i : i);
           uint64_t t0 = clock64();
           const bool eager = a.eager && a.eager[i];
           uint64_t nodes = solve_root(a, slot, stk, lds, me, canon, acc, s_hist, published, eager);

@@ -1,0 +1,658 @@
+// Packed search for gfx950: many small roots per wavefront, one tree node
+// per lane.
+//
+// The one-wave-per-root kernel (mce_kernel.hip, the reference's layout:
+// BkPivotBitBalance.cuh:510-608) spends a whole wave on one tree node at a
+// time, so a node of a root with |P0| <= 64 costs hundreds of wave
+// instructions for a few words of set work.  Here a wave keeps up to
+// pk_slots subproblems resident in its LDS and a stack of tree-node records
+// in HBM; every pass of its loop hands up to 64 pending tasks to the 64
+// lanes, and each lane computes one tree node on its own:
+//
+//   task    (record, candidate t of the record's V): the child of t as the
+//           sequential search makes it, P' = (P \ {v in V : v < t}) & N(t),
+//           XP' = (XP | {v in V : v < t}) & N(t), XX' = XX & cols(t)
+//           (BkPivotBitBalance.cuh:40-70: the k-th candidate of a published
+//           level, which is what the reference's thieves materialise),
+//           followed by aggressive node generation (:351-382);
+//   node    a leaf is counted when XP' and XX' are empty; otherwise the
+//           pivot is the (|P' \ N(u)|, lowest row) argmin over the rows u
+//           of P' | XP' | XX' (:229-257), and a node whose P' an excluded
+//           row covers is pruned; a branching node pushes its record, whose
+//           candidates V = P' \ N(pivot) become |V| tasks.
+//
+// Every child is a function of its parent record and its candidate alone,
+// so the order in which lanes take tasks never changes the search tree:
+// node counts, cliques, hash and histogram equal the one-wave-per-root
+// kernel's and the oracle's.  Dispatch takes tasks from the top of the stack
+// (the newest records), which keeps the stack near depth x 64 records;
+// beyond pk_thr records only the top record is expanded (plain depth-first
+// order), which bounds it by pk_thr + 64 (64 NW + 3) records.
+//
+// A subproblem occupies an LDS slot from its build until its last record is
+// consumed:  rows of P0 and of X0' over P0 (NW words), the columns of the X0'
+// block for each P0 vertex (NX words: XX' = XX & cols(t) is one load), and
+// the label hashes of P0.  Roots with |P0| > 64 NW or |X0'| > 64 NX are
+// appended to a.wide for the next launch of the run.
+#include "mce_common.hpp"
+
+extern __shared__ uint64_t lds_dyn[];
+
+namespace rdmce {
+namespace {
+
+constexpr uint32_t kRecRoot = 1u << 24;  // meta flag: the record's one task is its root's top node
+
+template <int NW, int NX>
+struct PackCfg {
+  static constexpr uint32_t RS = pack_rec_words(NW, NX);
+  static constexpr uint32_t SW = pack_slot_words(NW, NX);
+  static constexpr uint32_t HS = pack_hist_words(NW);
+  // record word offsets
+  static constexpr uint32_t oR = 0, oP = NW, oX = 2 * NW, oV = 3 * NW, oXX = 4 * NW, oKey = 4 * NW + NX,
+                            oMeta = 4 * NW + NX + 1, oRem = 4 * NW + NX + 2;
+  // slot word offsets
+  static constexpr uint32_t sProw = 0, sXrow = 64 * NW * NW, sXcol = sXrow + 64 * NX * NW, sG = sXcol + 64 * NW * NX;
+};
+
+// Position of the k-th (0-based) set bit of w; k < popcount(w).
+__device__ __forceinline__ uint32_t select64(uint64_t w, uint32_t k) {
+  uint32_t pos = 0, c;
+  c = (uint32_t)__popcll(w & 0xFFFFFFFFull);
+  if (k >= c) { k -= c; w >>= 32; pos = 32; }
+  c = (uint32_t)__popcll(w & 0xFFFFull);
+  if (k >= c) { k -= c; w >>= 16; pos += 16; }
+  c = (uint32_t)__popcll(w & 0xFFull);
+  if (k >= c) { k -= c; w >>= 8; pos += 8; }
+  c = (uint32_t)__popcll(w & 0xFull);
+  if (k >= c) { k -= c; w >>= 4; pos += 4; }
+  c = (uint32_t)__popcll(w & 0x3ull);
+  if (k >= c) { k -= c; w >>= 2; pos += 2; }
+  if (k >= (uint32_t)(w & 1)) pos += 1;
+  return pos;
+}
+
+// Sum over the wave of a per-lane 64-bit value (uniform result).
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+  uint64_t s = 0;
+  for (uint32_t k = 0; k < 64; ++k) s += readlane64(v, k);
+  return s;
+}
+
+// The per-wave LDS areas (word offsets from lds_dyn).
+struct PackLds {
+  ldsu64 *hist;     // kLdsHist clique sizes (outside batches)
+  lds64 *map;       // 64 u32: dispatch map
+  lds64 *meta;      // kPackMetaWords x S
+  lds64 *bhist;     // batches: HS x S per-slot clique sizes
+  lds64 *slots;     // SW x S
+};
+
+// Slot bookkeeping, kPackMetaWords arrays of S words.
+enum PackMeta { mCanon, mKey, mPX, mBase, mV, mNodes, mRefs, mCount, mHash };
+
+template <int NW, int NX>
+struct PackWave {
+  using C = PackCfg<NW, NX>;
+  const KArgs &a;
+  PackLds L;
+  uint32_t S;
+  uint32_t me;
+
+  __device__ __forceinline__ lds64 *m(PackMeta f, uint32_t s) const { return L.meta + (uint32_t)f * S + s; }
+  __device__ __forceinline__ lds64 *slot(uint32_t s) const { return L.slots + s * C::SW; }
+};
+
+// Counts one maximal clique of slot s (size, key = sum of label hashes),
+// with R's members given by rb (bits over P0).  Per lane.
+template <int NW, int NX>
+__device__ __forceinline__ void pk_emit(const PackWave<NW, NX> &w, uint32_t s, uint32_t size, uint64_t key,
+                                        const uint64_t (&rb)[NW], uint64_t &cnt, uint64_t &hsh) {
+  using C = PackCfg<NW, NX>;
+  const KArgs &a = w.a;
+  const uint64_t h = splitmix64_mix(key);
+  if (a.root_graph) {
+    __hip_atomic_fetch_add(w.m(mCount, s), (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(w.m(mHash, s), h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(w.L.bhist + s * C::HS + size, (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  } else {
+    cnt++;
+    hsh += h;
+    if (size < kLdsHist) {
+      __hip_atomic_fetch_add(w.L.hist + size, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else if (size < a.hist_len) {
+      atomicAdd(&a.hist[size], 1ull);
+    } else {
+      atomicAdd(&a.hist[a.hist_len - 1], 1ull);
+      atomicOr(a.err, kErrHistOverflow);
+    }
+  }
+  if (a.list) {
+    // entry: size, the root's vertices, then R's members in P0 order
+    const unsigned long long pos = atomicAdd(a.list_cursor, (unsigned long long)(size + 1));
+    if (pos + size + 1 > a.list_cap) {
+      atomicOr(a.err, kErrListFull);
+    } else {
+      uint32_t *dst = a.list + pos;
+      const uint64_t bg = *w.m(mBase, s), vv = *w.m(mV, s);
+      const uint32_t *pl = a.pk_plist + ((uint64_t)w.me * w.S + s) * kPackMaxP;
+      dst[0] = size;
+      dst[1] = a.labels[(uint32_t)vv];
+      uint32_t k = 1;
+      if ((uint32_t)bg == 2) dst[++k] = a.labels[(uint32_t)(vv >> 32)];
+      for (int q = 0; q < NW; ++q)
+        for (uint64_t r = rb[q]; r; r &= r - 1) dst[++k] = a.labels[pl[q * 64 + __builtin_ctzll(r)]];
+    }
+  }
+}
+
+// Builds root `canon` into slot s (the whole wave).  Returns 1 when the root
+// occupies the slot, 0 when it needs no search (its 2-clique, if maximal,
+// is counted here), 2 when it is too wide for this variant (appended to
+// a.wide).
+template <int NW, int NX>
+__device__ __forceinline__ uint32_t pk_build(const PackWave<NW, NX> &w, uint8_t *scratch, uint32_t s, uint64_t canon,
+                                             Acc &acc, uint64_t &cnt, uint64_t &hsh) {
+  using C = PackCfg<NW, NX>;
+  const KArgs &a = w.a;
+  const uint32_t l = lane();
+  Sub sub;
+  sub.canon = canon;
+  sub.graph = a.root_graph ? a.root_graph[canon] : 0;
+  uint32_t *Pl = (uint32_t *)scratch;
+  const uint64_t ccap = a.slot_bytes / 4;
+  uint64_t p = 0, x = 0;
+  const uint32_t above = root_lists(a, canon, Pl, ccap, sub, p, x, acc);
+  if (p == 0) {
+    if (x == 0 && l == 0) {  // the edge (or vertex) alone is a maximal clique
+      const uint64_t h = splitmix64_mix(sub.base_key);
+      if (a.root_graph) {
+        atomicAdd(&a.g_count[sub.graph], 1ull);
+        atomicAdd(&a.g_hash[sub.graph], h);
+        atomicAdd(&a.g_hist[(uint64_t)sub.graph * a.g_hist_stride + sub.base_size], 1ull);
+      } else {
+        cnt++;
+        hsh += h;
+        __hip_atomic_fetch_add(w.L.hist + sub.base_size, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      if (a.list) {
+        const unsigned long long pos = atomicAdd(a.list_cursor, (unsigned long long)(sub.base_size + 1));
+        if (pos + sub.base_size + 1 > a.list_cap) {
+          atomicOr(a.err, kErrListFull);
+        } else {
+          a.list[pos] = sub.base_size;
+          a.list[pos + 1] = a.labels[sub.v0];
+          if (sub.base_size == 2) a.list[pos + 2] = a.labels[sub.v1];
+        }
+      }
+    }
+    return 0;
+  }
+  if (p > 64u * NW || p + x > ccap) {
+    if (l == 0) a.wide[atomicAdd(a.n_wide, 1ull)] = canon;
+    return 2;
+  }
+  wave_fence();
+  const uint32_t *P = Pl;
+  uint32_t *X = Pl + p;
+  const uint64_t xq = compact_x(a, P, p, X, x, above);
+  wave_fence();
+  if (xq > 64u * NX) {
+    if (l == 0) a.wide[atomicAdd(a.n_wide, 1ull)] = canon;
+    return 2;
+  }
+  const uint64_t rows = p + xq;
+  // the bitmap tiers' accounting: both lists and a W x rows_pad bitmap
+  acc.build += 4 * (p + x) + 8ull * ((p + 63) / 64) * align_up(rows, 64);
+  lds64 *sl = w.slot(s);
+  for (uint32_t i = l; i < C::sXcol; i += 64) sl[i] = 0;  // P0 rows and X0' rows
+  for (uint32_t i = l; i < p; i += 64) sl[C::sG + i] = a.label_hash[P[i]];
+  if (a.list) {
+    uint32_t *pl = a.pk_plist + ((uint64_t)w.me * w.S + s) * kPackMaxP;
+    for (uint32_t i = l; i < p; i += 64) pl[i] = P[i];
+  }
+  wave_fence();
+  match_rows(
+      a, P, p, X, rows, above,
+      [&](uint64_t i, uint32_t j) {
+        or64_lds(sl + C::sProw + i * NW + (j >> 6), 1ull << (j & 63));
+        or64_lds(sl + C::sProw + (uint64_t)j * NW + (i >> 6), 1ull << (i & 63));
+      },
+      [&](uint64_t i, uint32_t wd, uint64_t bits) { or64_lds(sl + C::sXrow + (i - p) * NW + wd, bits); });
+  wave_fence();
+  // Columns of the X0' block: bit x of column j is bit j of row x.  Lane l
+  // holds row 64 b + l of block b; one ballot per P0 vertex and block.
+#pragma unroll
+  for (int b = 0; b < NX; ++b) {
+    uint64_t row[NW];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) row[q] = (64u * b + l < xq) ? (uint64_t)sl[C::sXrow + (64u * b + l) * NW + q] : 0ull;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      uint64_t mine = 0;
+      for (uint32_t j = 0; j < 64; ++j) {
+        const uint64_t col = __ballot((row[q] >> j) & 1);
+        if (l == j) mine = col;
+      }
+      if (64u * q + l < p) sl[C::sXcol + (64u * q + l) * NX + b] = mine;
+    }
+  }
+  if (l == 0) {
+    *w.m(mCanon, s) = canon;
+    *w.m(mKey, s) = sub.base_key;
+    *w.m(mPX, s) = p | (xq << 32);
+    *w.m(mBase, s) = (uint64_t)sub.base_size | ((uint64_t)sub.graph << 32);
+    *w.m(mV, s) = (uint64_t)sub.v0 | ((uint64_t)sub.v1 << 32);
+    *w.m(mNodes, s) = 0;
+    *w.m(mRefs, s) = 1;  // the root record pushed next
+    *w.m(mCount, s) = 0;
+    *w.m(mHash, s) = 0;
+  }
+  if (a.root_graph)
+    for (uint32_t i = l; i < C::HS; i += 64) w.L.bhist[s * C::HS + i] = 0;
+  wave_fence();
+  return 1;
+}
+
+// Moves what slot s accumulated (batches: count, hash, clique sizes; root
+// statistics: its node count) to the results.  Per lane (lane s).
+template <int NW, int NX>
+__device__ __forceinline__ void pk_finish(const PackWave<NW, NX> &w, uint32_t s) {
+  using C = PackCfg<NW, NX>;
+  const KArgs &a = w.a;
+  const uint64_t canon = *w.m(mCanon, s);
+  if (a.root_nodes) atomicAdd(&a.root_nodes[canon], (unsigned long long)*w.m(mNodes, s));
+  if (a.root_graph) {
+    const uint32_t g = (uint32_t)(*w.m(mBase, s) >> 32);
+    const uint64_t c = *w.m(mCount, s), h = *w.m(mHash, s);
+    if (c) atomicAdd(&a.g_count[g], (unsigned long long)c);
+    if (h) atomicAdd(&a.g_hash[g], (unsigned long long)h);
+    for (uint32_t i = 0; i < C::HS; ++i) {
+      const uint64_t v = w.L.bhist[s * C::HS + i];
+      if (v) {
+        if (i < a.g_hist_stride) atomicAdd(&a.g_hist[(uint64_t)g * a.g_hist_stride + i], (unsigned long long)v);
+        else atomicOr(a.err, kErrHistOverflow);
+      }
+    }
+  }
+}
+
+template <int NW, int NX>
+__device__ __forceinline__ void pack_main(const KArgs &a) {
+  using C = PackCfg<NW, NX>;
+  const uint32_t l = lane();
+  const uint32_t S = a.pk_slots;
+  lds64 *base = (lds64 *)lds_dyn;
+  PackWave<NW, NX> w{a, {}, S, (uint32_t)blockIdx.x};
+  w.L.hist = (ldsu64 *)base;
+  w.L.map = base + kLdsHist;
+  w.L.meta = w.L.map + 32;
+  w.L.bhist = w.L.meta + kPackMetaWords * S;
+  w.L.slots = w.L.bhist + (a.root_graph ? C::HS * S : 0);
+  for (uint32_t i = l; i < kLdsHist; i += 64) w.L.hist[i] = 0;
+  wave_fence();
+
+  const uint64_t n_dispatch = a.n_dispatch_dev ? ld_relaxed((unsigned long long *)a.n_dispatch_dev) : a.n_dispatch;
+  uint8_t *scratch = a.slots + (uint64_t)w.me * a.slot_bytes;
+  uint64_t *stk = a.pk_stack + (uint64_t)w.me * a.pk_cap * C::RS;
+  ldsu32 *map32 = (ldsu32 *)w.L.map;
+  Acc acc{0, 0, 0, 0};
+  uint64_t cnt = 0, hsh = 0, words = 0, nodes = 0;  // per lane
+  uint32_t top = 0;                                  // records in the stack
+  uint64_t pending = 0;                              // candidates (tasks) in the stack
+  uint32_t busy = 0;                                 // occupied slots
+  const uint32_t full = S >= 32 ? ~0u : ((1u << S) - 1);
+  bool roots_left = true;
+  for (;;) {
+    // Refill: build roots into free slots while fewer than 64 tasks wait.
+    while (roots_left && pending < 64 && busy != full) {
+      unsigned long long i = 0;
+      if (l == 0) i = atomicAdd(a.next_root, 1ull);
+      i = bcast64(i);
+      if (i >= n_dispatch) {
+        roots_left = false;
+        break;
+      }
+      const uint64_t canon = a.perm ? a.perm[i] : (a.reverse ? n_dispatch - 1 - i : i);
+      const uint32_t s = (uint32_t)__builtin_ctzll((uint64_t)(~busy & full));
+      if (pk_build<NW, NX>(w, scratch, s, canon, acc, cnt, hsh) == 1) {
+        if (l == 0) {
+          uint64_t *rec = stk + (uint64_t)top * C::RS;
+          for (uint32_t q = 0; q < C::RS; ++q) rec[q] = 0;
+          rec[C::oMeta] = ((uint64_t)s << 16) | kRecRoot;
+          rec[C::oRem] = 1;
+        }
+        wave_fence();
+        ++top;
+        ++pending;
+        busy |= 1u << s;
+      }
+    }
+    if (top == 0) break;  // no records and no roots left (see the refill condition)
+
+    // Dispatch up to 64 tasks from the top of the stack: look-lane i sees
+    // record top-1-i; task lane l takes candidate k of look-record ri.
+    const uint32_t nlook = top < 64 ? top : 64;
+    uint32_t ci = 0;
+    uint64_t lrem[NW];
+    uint32_t lslot = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) lrem[q] = 0;
+    if (l < nlook) {
+      const uint64_t *rec = stk + (uint64_t)(top - 1 - l) * C::RS;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        lrem[q] = rec[C::oRem + q];
+        ci += (uint32_t)__popcll(lrem[q]);
+      }
+      lslot = (uint32_t)(rec[C::oMeta] >> 16) & 0xFF;
+    }
+    const uint32_t incl = wave_scan_add_u32(ci), excl = incl - ci;
+    uint32_t T = __builtin_amdgcn_readlane(incl, 63);  // tasks in the top 64 records: min(pending, 64) or more
+    T = T < 64 ? T : 64u;
+    if (top > a.pk_thr) {  // deep stack: expand the top record only
+      const uint32_t c0 = __builtin_amdgcn_readlane(ci, 0);
+      T = T < c0 ? T : c0;
+    }
+    const bool lfull = l < nlook && incl <= T;
+    const bool lpart = l < nlook && excl < T && incl > T;
+    const uint32_t nfull = (uint32_t)__popcll(__ballot(lfull));
+    map32[l] = 0;
+    wave_fence();
+    if (l < nlook && excl < T) map32[excl] = ((l + 1) << 8) | excl;
+    wave_fence();
+    const uint32_t mv = wave_scan_max_u32(l < T ? map32[l] : 0u);
+    const bool act = l < T;
+    const uint32_t ri = act ? (mv >> 8) - 1 : 0u, k = act ? l - (mv & 0xFF) : 0u;
+
+    // The task's parent record.
+    uint64_t R[NW], P[NW], XP[NW], V[NW], XX[NX], key = 0;
+    uint32_t meta = 0, t = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) R[q] = P[q] = XP[q] = V[q] = 0;
+#pragma unroll
+    for (int j = 0; j < NX; ++j) XX[j] = 0;
+    if (act) {
+      const uint64_t *rec = stk + (uint64_t)(top - 1 - ri) * C::RS;
+      uint64_t rem[NW];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        R[q] = rec[C::oR + q];
+        P[q] = rec[C::oP + q];
+        XP[q] = rec[C::oX + q];
+        V[q] = rec[C::oV + q];
+        rem[q] = rec[C::oRem + q];
+      }
+#pragma unroll
+      for (int j = 0; j < NX; ++j) XX[j] = rec[C::oXX + j];
+      key = rec[C::oKey];
+      meta = (uint32_t)rec[C::oMeta];
+      // t = the k-th remaining candidate
+      uint32_t kk = k, base = 0;
+#pragma unroll
+      for (int q = 0; q < NW - 1; ++q) {
+        const uint32_t cq = (uint32_t)__popcll(rem[q]);
+        if (base == (uint32_t)q * 64 && kk >= cq) {
+          kk -= cq;
+          base = (uint32_t)(q + 1) * 64;
+        }
+      }
+      uint64_t rw = rem[0];
+#pragma unroll
+      for (int q = 1; q < NW; ++q)
+        if (base == (uint32_t)q * 64) rw = rem[q];
+      t = base + select64(rw, kk);
+    }
+    wave_fence();  // every parent is read before records are rewritten below
+    if (lpart) {   // the partly taken record keeps its candidates from the (T - excl)-th on
+      uint64_t *rec = stk + (uint64_t)(top - 1 - l) * C::RS;
+      uint32_t kk = T - excl, base = 0;
+#pragma unroll
+      for (int q = 0; q < NW - 1; ++q) {
+        const uint32_t cq = (uint32_t)__popcll(lrem[q]);
+        if (base == (uint32_t)q * 64 && kk >= cq) {
+          kk -= cq;
+          base = (uint32_t)(q + 1) * 64;
+        }
+      }
+      uint64_t rw = lrem[0];
+#pragma unroll
+      for (int q = 1; q < NW; ++q)
+        if (base == (uint32_t)q * 64) rw = lrem[q];
+      const uint32_t first = base + select64(rw, kk);
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        const uint32_t lo = 64u * q;
+        const uint64_t keep = first <= lo ? ~0ull : (first >= lo + 64 ? 0ull : ~0ull << (first - lo));
+        rec[C::oRem + q] = lrem[q] & keep;
+      }
+    }
+    if (lfull) __hip_atomic_fetch_sub(w.m(mRefs, lslot), (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    top -= nfull;
+
+    // The task: one tree node per lane.
+    const uint32_t s = (meta >> 16) & 0xFF;
+    bool branch = false;
+    if (act) {
+      const lds64 *sl = w.slot(s);
+      const uint64_t px = *w.m(mPX, s);
+      const uint32_t p = (uint32_t)px, xq = (uint32_t)(px >> 32);
+      const uint32_t Wr = (p + 63) / 64;  // the root's own width (accounting as the other kernel)
+      uint32_t rsize;
+      if (meta & kRecRoot) {  // the root's top node: R = {}, P = P0, XP = {}, XX = X0'
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+          R[q] = 0;
+          P[q] = low_mask(p > 64u * q ? p - 64u * q : 0u);
+          XP[q] = 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NX; ++j) XX[j] = low_mask(xq > 64u * j ? xq - 64u * j : 0u);
+        key = *w.m(mKey, s);
+        rsize = 0;
+      } else {
+        // Child of candidate t: the parent's candidates below t have moved
+        // from P to XP, as when the sequential search reaches t.
+        const uint32_t jt = t >> 6, bt = t & 63;
+        uint32_t xpop = 0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+          const uint64_t below = (uint32_t)q < jt ? V[q] : ((uint32_t)q == jt ? V[q] & low_mask(bt) : 0ull);
+          const uint64_t rt = sl[C::sProw + t * NW + q];
+          P[q] = P[q] & ~below & rt;
+          XP[q] = (XP[q] | below) & rt;
+          R[q] |= (uint32_t)q == jt ? 1ull << bt : 0ull;
+        }
+#pragma unroll
+        for (int j = 0; j < NX; ++j) {
+          xpop += (uint32_t)__popcll(XX[j]);
+          XX[j] &= sl[C::sXcol + t * NX + j];
+        }
+        key += sl[C::sG + t];
+        rsize = (meta & 0xFFFF) + 1;
+        words += Wr;
+        // Aggressive node generation: every vertex of P' adjacent to all of
+        // P' \ {itself} moves to R; XP' and XX' keep what is adjacent to it.
+        uint32_t psz = 0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) psz += (uint32_t)__popcll(P[q]);
+        bool anym = false;
+        if (psz > 1) {
+          words += (uint64_t)psz * Wr;
+          uint64_t mk[NW], xacc[NW];
+#pragma unroll
+          for (int q = 0; q < NW; ++q) {
+            mk[q] = 0;
+            xacc[q] = ~0ull;
+          }
+#pragma unroll
+          for (int q = 0; q < NW; ++q) {
+            for (uint64_t m = P[q]; m; m &= m - 1) {
+              const uint32_t i = 64u * q + (uint32_t)__builtin_ctzll(m);
+              uint64_t ri[NW];
+              uint32_t c = 0;
+#pragma unroll
+              for (int z = 0; z < NW; ++z) {
+                ri[z] = sl[C::sProw + i * NW + z];
+                c += (uint32_t)__popcll(P[z] & ~ri[z]);
+              }
+              if (c == 1) {
+                mk[q] |= m & (~m + 1);
+#pragma unroll
+                for (int z = 0; z < NW; ++z) xacc[z] &= ri[z];
+              }
+            }
+          }
+          uint32_t nm = 0;
+#pragma unroll
+          for (int q = 0; q < NW; ++q) nm += (uint32_t)__popcll(mk[q]);
+          if (nm) {
+            anym = true;
+#pragma unroll
+            for (int q = 0; q < NW; ++q) {
+              R[q] |= mk[q];
+              P[q] &= ~mk[q];
+              XP[q] &= xacc[q];
+              for (uint64_t m = mk[q]; m; m &= m - 1) {
+                const uint32_t i = 64u * q + (uint32_t)__builtin_ctzll(m);
+#pragma unroll
+                for (int j = 0; j < NX; ++j) XX[j] &= sl[C::sXcol + i * NX + j];
+                key += sl[C::sG + i];
+              }
+            }
+            rsize += nm;
+            words += (uint64_t)nm * Wr;
+          }
+        }
+        words += (uint64_t)xpop * (anym ? Wr + 1 : 1);
+      }
+      // Evaluate the node.
+      uint32_t psz = 0, cw = 0, xw = 0;
+      uint64_t xany = 0;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        psz += (uint32_t)__popcll(P[q]);
+        cw += (uint32_t)__popcll(P[q] | XP[q]);
+        xany |= XP[q];
+      }
+#pragma unroll
+      for (int j = 0; j < NX; ++j) {
+        xw += (uint32_t)__popcll(XX[j]);
+        xany |= XX[j];
+      }
+      ++nodes;
+      if (a.root_nodes) __hip_atomic_fetch_add(w.m(mNodes, s), (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (psz == 0) {
+        if (xany == 0) pk_emit<NW, NX>(w, s, (uint32_t)*w.m(mBase, s) + rsize, key, R, cnt, hsh);
+      } else {
+        words += (uint64_t)(cw + xw) * Wr;
+        // pivot: (|P' \ N(u)|, row) argmin; rows of P0 first, then X0'
+        uint32_t best = ~0u;
+        bool zero = false;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+          for (uint64_t m = P[q] | XP[q]; m && !zero; m &= m - 1) {
+            const uint32_t u = 64u * q + (uint32_t)__builtin_ctzll(m);
+            uint32_t c = 0;
+#pragma unroll
+            for (int z = 0; z < NW; ++z) c += (uint32_t)__popcll(P[z] & ~(uint64_t)sl[C::sProw + u * NW + z]);
+            const uint32_t kx = (c << 16) | u;
+            best = kx < best ? kx : best;
+            zero = c == 0;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < NX; ++j) {
+          for (uint64_t m = XX[j]; m && !zero; m &= m - 1) {
+            const uint32_t x = 64u * j + (uint32_t)__builtin_ctzll(m);
+            uint32_t c = 0;
+#pragma unroll
+            for (int z = 0; z < NW; ++z) c += (uint32_t)__popcll(P[z] & ~(uint64_t)sl[C::sXrow + x * NW + z]);
+            const uint32_t kx = (c << 16) | (p + x);
+            best = kx < best ? kx : best;
+            zero = c == 0;
+          }
+        }
+        if (!zero) {  // else an excluded row covers P': no maximal clique below
+          words += Wr;
+          const uint32_t piv = best & 0xFFFF;
+          const lds64 *pr = piv < p ? sl + C::sProw + piv * NW : sl + C::sXrow + (piv - p) * NW;
+#pragma unroll
+          for (int q = 0; q < NW; ++q) V[q] = P[q] & ~(uint64_t)pr[q];
+          branch = true;
+        }
+      }
+      meta = (meta & 0x00FF0000u) | rsize;
+    }
+
+    // Push the branching nodes' records (the children of the deepest
+    // records highest) and account.
+    uint64_t bm = __ballot(branch);
+    if (top + (uint32_t)__popcll(bm) > a.pk_cap) {  // unreachable by the bound above; never write past the arena
+      if (l == 0) atomicOr(a.err, kErrPackStack);
+      bm = 0;
+      branch = false;
+    }
+    const uint32_t npush = (uint32_t)__popcll(bm);
+    uint32_t vc = 0;
+    if (branch) {
+      uint64_t *rec = stk + (uint64_t)(top + npush - 1 - rank_below(bm)) * C::RS;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        rec[C::oR + q] = R[q];
+        rec[C::oP + q] = P[q];
+        rec[C::oX + q] = XP[q];
+        rec[C::oV + q] = V[q];
+        rec[C::oRem + q] = V[q];
+        vc += (uint32_t)__popcll(V[q]);
+      }
+#pragma unroll
+      for (int j = 0; j < NX; ++j) rec[C::oXX + j] = XX[j];
+      rec[C::oKey] = key;
+      rec[C::oMeta] = meta;
+      __hip_atomic_fetch_add(w.m(mRefs, s), (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    const uint32_t vsum = __builtin_amdgcn_readlane(wave_scan_add_u32(vc), 63);
+    top += npush;
+    pending = pending - T + vsum;
+    wave_fence();
+    // Slots whose last record is consumed are done.
+    const bool done = l < S && ((busy >> l) & 1) && *w.m(mRefs, l) == 0;
+    const uint64_t dm = __ballot(done);
+    if (done) pk_finish<NW, NX>(w, l);
+    busy &= ~(uint32_t)dm;
+  }
+
+  // Exit: the wave's sums.
+  const uint64_t wc = wave_sum64(cnt), wh = wave_sum64(hsh), ww = wave_sum64(words), wn = wave_sum64(nodes);
+  if (l == 0) {
+    if (wc) atomicAdd(a.count, (unsigned long long)wc);
+    if (wh) atomicAdd(a.hash, (unsigned long long)wh);
+    if (wn) atomicAdd(a.total_nodes, (unsigned long long)wn);
+    if (ww) atomicAdd(a.search_words, (unsigned long long)ww);
+    if (acc.build) atomicAdd(a.build_bytes, acc.build);
+  }
+  wave_fence();
+  for (uint32_t i = l; i < kLdsHist && i < a.hist_len; i += 64)
+    if (w.L.hist[i]) atomicAdd(&a.hist[i], (unsigned long long)w.L.hist[i]);
+}
+
+template <int NW, int NX>
+__global__ __launch_bounds__(64) void pack_kernel(KArgs a) {
+  pack_main<NW, NX>(a);
+}
+
+}  // namespace
+
+hipError_t launch_pack(const KArgs &a, int variant, uint32_t waves, hipStream_t stream) {
+  const bool batch = a.root_graph != nullptr;
+  if (variant == 0)
+    hipLaunchKernelGGL((pack_kernel<1, 1>), dim3(waves), dim3(64), pack_lds_bytes(1, 1, a.pk_slots, batch), stream,
+                       a);
+  else
+    hipLaunchKernelGGL((pack_kernel<2, 2>), dim3(waves), dim3(64), pack_lds_bytes(2, 2, a.pk_slots, batch), stream,
+                       a);
+  return hipGetLastError();
+}
+
+}  // namespace rdmce

@@ -51,7 +51,8 @@ class MceOutput:
 def make_options(device: int = 0, root_mode: int = _lib.ROOT_EDGE, waves_per_cu: int = 0, share_bound: int = 0,
                  want_list: bool = False, want_root_stats: bool = False, list_capacity: int = 0,
                  slot_bytes: int = 0, root_order=None, lds_tier: int = 1, use_graph: bool = False,
-                 lds_bytes: int = 0, eager=None, word_path: bool = True) -> tuple[MceOptions, object]:
+                 lds_bytes: int = 0, eager=None, word_path: bool = True, pack: bool = False, pack_slots: int = 0,
+                 pack_waves_per_cu: int = 0) -> tuple[MceOptions, object]:
     o = MceOptions()
     lib().mce_options_init(byref(o))
     o.device = device
@@ -66,6 +67,9 @@ def make_options(device: int = 0, root_mode: int = _lib.ROOT_EDGE, waves_per_cu:
     o.use_graph = int(use_graph)
     o.lds_bytes = lds_bytes
     o.word_path = 0 if word_path else -1
+    o.pack = int(pack)
+    o.pack_slots = pack_slots
+    o.pack_waves_per_cu = pack_waves_per_cu
     keep = []
     if root_order is not None:
         ro = np.ascontiguousarray(root_order, dtype=np.uint64)
