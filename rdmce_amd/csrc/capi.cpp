@@ -427,6 +427,10 @@ struct mce_plan {
   uint32_t *d_pk_plist = nullptr;
   uint64_t *d_wide[2] = {nullptr, nullptr};
   uint64_t wide_cap = 0;
+  std::vector<uint64_t> h_rfirst;    // edge roots: canonical id of the first root of each vertex, + m
+  uint64_t *d_rfirst = nullptr;
+  uint64_t *d_gstart = nullptr;      // runs of the plan's dispatch list sharing their first vertex
+  uint64_t n_groups = 0;
   // captured run (hipGraph)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -461,7 +465,8 @@ void plan_release(mce_plan *p) {
                   p->d_root_nodes, p->d_root_cycles, p->d_list,     p->d_deferred,    p->d_deferred2,
                   p->d_slots2,   p->d_stacks,      p->d_stacks2,    p->d_recs,        p->d_users,
                   p->d_has_work, p->d_root_graph,  p->d_gcount,     p->d_ghash,       p->d_ghist,
-                  p->d_idle,     p->d_pk_stack,    p->d_pk_plist,   p->d_wide[0],     p->d_wide[1]};
+                  p->d_idle,     p->d_pk_stack,    p->d_pk_plist,   p->d_wide[0],     p->d_wide[1],
+                  p->d_rfirst,   p->d_gstart};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &e : p->ev)
@@ -501,6 +506,27 @@ void set_config(mce_plan *p, int wpc, int share_bound, int lds_tier) {
   p->lds_tier = lds_tier;
   p->lds_bytes = lds_for(p, wpc, p->opt.lds_bytes);
   drop_graph(p);
+}
+
+// Runs of consecutive dispatch entries that are edge roots of one first
+// vertex (the packed search builds that vertex's bitmap once per run):
+// run r covers dispatch entries [out[r], out[r + 1]).  disp(i) gives the
+// canonical root id of dispatch entry i.
+template <class F>
+std::vector<uint64_t> dispatch_runs(const std::vector<uint64_t> &rfirst, uint64_t n_disp, F disp) {
+  std::vector<uint64_t> out;
+  out.push_back(0);
+  uint64_t lo = 1, hi = 0;  // canonical ids of the current first vertex: [lo, hi)
+  for (uint64_t i = 0; i < n_disp; ++i) {
+    const uint64_t r = disp(i);
+    if (r >= lo && r < hi) continue;
+    const uint64_t v0 = (uint64_t)(std::upper_bound(rfirst.begin(), rfirst.end(), r) - rfirst.begin()) - 1;
+    lo = rfirst[v0];
+    hi = rfirst[v0 + 1];
+    if (i) out.push_back(i);
+  }
+  if (n_disp) out.push_back(n_disp);
+  return out;
 }
 
 // What a plan is built from: a (possibly concatenated) graph, and for
@@ -594,6 +620,24 @@ mce_status plan_create_impl(const PlanSpec &spec, const mce_options &opt, mce_pl
     p->n_dispatch = opt.n_roots;
     PTRY(dalloc(&p->d_perm, opt.n_roots));
     if (opt.n_roots) PTRY(hipMemcpy(p->d_perm, opt.roots, opt.n_roots * 8, hipMemcpyHostToDevice));
+  }
+  if (mode == MCE_ROOT_EDGE) {
+    p->h_rfirst.assign(g.n + 1, 0);
+    uint64_t r = 0;
+    for (uint64_t v = 0; v < g.n; ++v) {
+      p->h_rfirst[v] = r;
+      while (r < p->n_roots && (roots[r] >> 32) == v) ++r;
+    }
+    p->h_rfirst[g.n] = r;
+    PTRY(dalloc(&p->d_rfirst, g.n + 1));
+    PTRY(hipMemcpy(p->d_rfirst, p->h_rfirst.data(), (g.n + 1) * 8, hipMemcpyHostToDevice));
+    const uint64_t nd = p->n_dispatch;
+    const uint64_t *cr = opt.schedule == MCE_SCHED_CUSTOM ? opt.roots : nullptr;
+    std::vector<uint64_t> runs =
+        dispatch_runs(p->h_rfirst, nd, [&](uint64_t i) { return cr ? cr[i] : nd - 1 - i; });
+    p->n_groups = runs.size() - 1;
+    PTRY(dalloc(&p->d_gstart, runs.size()));
+    PTRY(hipMemcpy(p->d_gstart, runs.data(), runs.size() * 8, hipMemcpyHostToDevice));
   }
   if (opt.eager) {
     PTRY(dalloc(&p->d_eager, p->n_dispatch));
@@ -709,6 +753,9 @@ KArgs make_args(const mce_plan *p) {
   a.g_hash = p->d_ghash;
   a.g_hist = p->d_ghist;
   a.g_hist_stride = p->g_hist_stride;
+  a.rfirst = p->d_rfirst;
+  a.pk_gstart = p->d_gstart;
+  a.pk_ngroups = p->n_groups;
   return a;
 }
 
@@ -784,6 +831,7 @@ hipError_t enqueue_run(const mce_plan *p, const KArgs &a, hipStream_t s) {
     b.n_wide = p->d_ctr + (v == 0 ? kNWide1 : kNWide2);
     b.eager = nullptr;
     if (v == 1) {
+      b.pk_gstart = nullptr;
       b.perm = p->d_wide[0];
       b.reverse = 0;
       b.n_dispatch_dev = p->d_ctr + kNWide1;
@@ -987,9 +1035,16 @@ mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
     for (uint64_t i = 0; i < p->n_dispatch; ++i) disp[i] = p->n_dispatch - 1 - i;
   std::vector<uint64_t> sample;
   for (uint64_t i = 0; i < disp.size(); i += stride) sample.push_back(disp[i]);
-  uint64_t *d_sample = nullptr;
+  uint64_t *d_sample = nullptr, *d_sruns = nullptr;
   HIP_TRY(dalloc(&d_sample, sample.size()));
   if (!sample.empty()) HIP_TRY(hipMemcpy(d_sample, sample.data(), sample.size() * 8, hipMemcpyHostToDevice));
+  uint64_t n_sruns = 0;
+  if (!p->h_rfirst.empty()) {
+    std::vector<uint64_t> runs = dispatch_runs(p->h_rfirst, sample.size(), [&](uint64_t i) { return sample[i]; });
+    n_sruns = runs.size() - 1;
+    HIP_TRY(dalloc(&d_sruns, runs.size()));
+    HIP_TRY(hipMemcpy(d_sruns, runs.data(), runs.size() * 8, hipMemcpyHostToDevice));
+  }
   // Waves per CU first (LDS tier on), then the sharing bound at the best
   // occupancy, then waves per CU again when the bound changed, then the LDS
   // tier off at the best pair.
@@ -1006,6 +1061,8 @@ mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
     a.eager = nullptr;
     a.n_dispatch = sample.size();
     a.reverse = 0;
+    a.pk_gstart = d_sruns;
+    a.pk_ngroups = n_sruns;
     RunOut ro;
     mce_status s = run_core(p, a, false, ro);
     if (s) return s;
@@ -1065,6 +1122,7 @@ mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
     }
   }
   (void)hipFree(d_sample);
+  if (d_sruns) (void)hipFree(d_sruns);
   if (st) {
     set_config(p, base_wpc, base_tau, base_lds);
     return st;

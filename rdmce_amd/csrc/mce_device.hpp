@@ -128,6 +128,12 @@ struct KArgs {
   uint32_t *pk_plist;               // per wave pk_slots * kPackMaxP P0 vertex ids (clique lists)
   uint64_t *wide;
   unsigned long long *n_wide;
+  // Edge roots in runs sharing their first vertex: run r covers dispatch
+  // entries [pk_gstart[r], pk_gstart[r + 1]) (nullable: single entries);
+  // rfirst[v] = canonical id of the first edge root of vertex v.
+  const uint64_t *pk_gstart;
+  uint64_t pk_ngroups;
+  const uint64_t *rfirst;
 };
 
 // Packed search variants: |P0| <= 64 NW and |X0'| <= 64 NX.

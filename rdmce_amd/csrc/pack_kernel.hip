@@ -146,6 +146,79 @@ __device__ __forceinline__ void pk_emit(const PackWave<NW, NX> &w, uint32_t s, u
   }
 }
 
+// Completes slot s once its P0 rows, X0' rows, label hashes (and clique-list
+// ids) are in place: the columns of the X0' block and the bookkeeping.
+template <int NW, int NX>
+__device__ __forceinline__ void pk_seal(const PackWave<NW, NX> &w, uint32_t s, uint64_t p, uint64_t xq, uint64_t canon,
+                                        uint64_t base_key, uint32_t base_size, uint32_t graph, uint32_t v0,
+                                        uint32_t v1) {
+  using C = PackCfg<NW, NX>;
+  const KArgs &a = w.a;
+  const uint32_t l = lane();
+  lds64 *sl = w.slot(s);
+  wave_fence();
+  // Columns of the X0' block: bit x of column j is bit j of row x.  Lane l
+  // holds row 64 b + l of block b; one ballot per P0 vertex and block.
+#pragma unroll
+  for (int b = 0; b < NX; ++b) {
+    uint64_t row[NW];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) row[q] = (64u * b + l < xq) ? (uint64_t)sl[C::sXrow + (64u * b + l) * NW + q] : 0ull;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      uint64_t mine = 0;
+      for (uint32_t j = 0; j < 64; ++j) {
+        const uint64_t col = __ballot((row[q] >> j) & 1);
+        if (l == j) mine = col;
+      }
+      if (64u * q + l < p) sl[C::sXcol + (64u * q + l) * NX + b] = mine;
+    }
+  }
+  if (l == 0) {
+    *w.m(mCanon, s) = canon;
+    *w.m(mKey, s) = base_key;
+    *w.m(mPX, s) = p | (xq << 32);
+    *w.m(mBase, s) = (uint64_t)base_size | ((uint64_t)graph << 32);
+    *w.m(mV, s) = (uint64_t)v0 | ((uint64_t)v1 << 32);
+    *w.m(mNodes, s) = 0;
+    *w.m(mRefs, s) = 1;  // the root record pushed next
+    *w.m(mCount, s) = 0;
+    *w.m(mHash, s) = 0;
+  }
+  if (a.root_graph)
+    for (uint32_t i = l; i < C::HS; i += 64) w.L.bhist[s * C::HS + i] = 0;
+  wave_fence();
+}
+
+// Counts the root's own vertex set {v0, v1} (or {v0}) as a maximal clique:
+// a root whose candidate and excluded sets are both empty.  Lane 0.
+template <int NW, int NX>
+__device__ __forceinline__ void pk_emit_base(const PackWave<NW, NX> &w, uint64_t base_key, uint32_t base_size,
+                                             uint32_t graph, uint32_t v0, uint32_t v1, uint64_t &cnt, uint64_t &hsh) {
+  const KArgs &a = w.a;
+  if (lane() != 0) return;
+  const uint64_t h = splitmix64_mix(base_key);
+  if (a.root_graph) {
+    atomicAdd(&a.g_count[graph], 1ull);
+    atomicAdd(&a.g_hash[graph], h);
+    atomicAdd(&a.g_hist[(uint64_t)graph * a.g_hist_stride + base_size], 1ull);
+  } else {
+    cnt++;
+    hsh += h;
+    __hip_atomic_fetch_add(w.L.hist + base_size, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  if (a.list) {
+    const unsigned long long pos = atomicAdd(a.list_cursor, (unsigned long long)(base_size + 1));
+    if (pos + base_size + 1 > a.list_cap) {
+      atomicOr(a.err, kErrListFull);
+    } else {
+      a.list[pos] = base_size;
+      a.list[pos + 1] = a.labels[v0];
+      if (base_size == 2) a.list[pos + 2] = a.labels[v1];
+    }
+  }
+}
+
 // Builds root `canon` into slot s (the whole wave).  Returns 1 when the root
 // occupies the slot, 0 when it needs no search (its 2-clique, if maximal,
 // is counted here), 2 when it is too wide for this variant (appended to
@@ -164,28 +237,7 @@ __device__ __forceinline__ uint32_t pk_build(const PackWave<NW, NX> &w, uint8_t 
   uint64_t p = 0, x = 0;
   const uint32_t above = root_lists(a, canon, Pl, ccap, sub, p, x, acc);
   if (p == 0) {
-    if (x == 0 && l == 0) {  // the edge (or vertex) alone is a maximal clique
-      const uint64_t h = splitmix64_mix(sub.base_key);
-      if (a.root_graph) {
-        atomicAdd(&a.g_count[sub.graph], 1ull);
-        atomicAdd(&a.g_hash[sub.graph], h);
-        atomicAdd(&a.g_hist[(uint64_t)sub.graph * a.g_hist_stride + sub.base_size], 1ull);
-      } else {
-        cnt++;
-        hsh += h;
-        __hip_atomic_fetch_add(w.L.hist + sub.base_size, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-      if (a.list) {
-        const unsigned long long pos = atomicAdd(a.list_cursor, (unsigned long long)(sub.base_size + 1));
-        if (pos + sub.base_size + 1 > a.list_cap) {
-          atomicOr(a.err, kErrListFull);
-        } else {
-          a.list[pos] = sub.base_size;
-          a.list[pos + 1] = a.labels[sub.v0];
-          if (sub.base_size == 2) a.list[pos + 2] = a.labels[sub.v1];
-        }
-      }
-    }
+    if (x == 0) pk_emit_base(w, sub.base_key, sub.base_size, sub.graph, sub.v0, sub.v1, cnt, hsh);
     return 0;
   }
   if (p > 64u * NW || p + x > ccap) {
@@ -220,37 +272,236 @@ __device__ __forceinline__ uint32_t pk_build(const PackWave<NW, NX> &w, uint8_t 
       },
       [&](uint64_t i, uint32_t wd, uint64_t bits) { or64_lds(sl + C::sXrow + (i - p) * NW + wd, bits); });
   wave_fence();
-  // Columns of the X0' block: bit x of column j is bit j of row x.  Lane l
-  // holds row 64 b + l of block b; one ballot per P0 vertex and block.
+  pk_seal<NW, NX>(w, s, p, xq, canon, sub.base_key, sub.base_size, sub.graph, sub.v0, sub.v1);
+  return 1;
+}
+
+// ---------------------------------------------------------------------------
+// Edge roots derived from their first vertex's bitmap.
+//
+// Every edge root (v0, u_k) with u_k the k-th vertex of U = N+(v0) is a
+// restriction of the vertex root v0 (BkPivotBitBalance.cuh:414-427 builds it
+// as the LEVEL_1 root): P0 = {u_j : j > k, u_j ~ u_k}, X0 = {x ~ v0 : x < v0,
+// x ~ u_k} then {u_j : j < k, u_j ~ u_k} (ascending vertex order), and X0'
+// keeps the rows of X0 with a neighbour in P0.  A wave that takes a run of
+// edge roots of one v0 therefore builds the vertex root's bitmap once, in its
+// scratch region (rows U then X0v' = the x ~ v0 below v0 with a neighbour in
+// U, word-major), and derives each edge root from it by bit extraction:
+// rows of the edge root are the vertex rows restricted to P0's columns and
+// compressed to P0's indices.  The subproblem (rows, order, label hashes) is
+// exactly the one pk_build makes, so the search tree is unchanged.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kGroupMaxW = 16;  // |U| <= 1024
+
+struct PkGroup {
+  uint32_t v0, d0, W0, xq0, rp;
+  uint64_t o0;          // U = col[o0, o0 + d0)
+  uint64_t rfirst;      // canonical id of the edge root (v0, U[0])
+  const uint64_t *T;    // W0 x rp words: rows 0..d0-1 (U), d0..d0+xq0-1 (X0v')
+  const uint64_t *cm;   // W0 words: OR of the X0v' rows (U vertices with a neighbour below v0)
+};
+
+// Builds the bitmap of vertex root v0 into `scratch` (the whole wave).
+// Returns false when it does not fit (the caller then builds each edge root
+// on its own).
+__device__ __forceinline__ bool pk_group_build(const KArgs &a, uint8_t *scratch, uint32_t v0, PkGroup &G, Acc &acc) {
+  const uint32_t l = lane();
+  const uint64_t b0 = a.rowoff[v0], e0 = a.rowoff[(uint64_t)v0 + 1], o0 = a.outoff[v0];
+  const uint64_t d0 = e0 - o0, x0 = o0 - b0;
+  const uint64_t W0 = (d0 + 63) / 64;
+  if (W0 == 0 || W0 > kGroupMaxW) return false;
+  const uint64_t t_off = align_up(4 * x0, 256);
+  const uint64_t rp_max = align_up(d0 + x0, 64);
+  const uint64_t cm_off = t_off + align_up(8 * W0 * rp_max, 256);
+  if (cm_off + 8 * W0 > a.slot_bytes) return false;
+  uint32_t *Xs = (uint32_t *)scratch;
+  for (uint64_t i = l; i < x0; i += 64) Xs[i] = a.col[b0 + i];
+  wave_fence();
+  const uint32_t *U = a.col + o0;
+  const uint64_t xq0 = compact_x(a, U, d0, Xs, x0, v0);
+  wave_fence();
+  const uint64_t rows = d0 + xq0, rp = align_up(rows, 64);
+  uint64_t *T = (uint64_t *)(scratch + t_off);
+  uint64_t *cm = (uint64_t *)(scratch + cm_off);
+  for (uint64_t i = l; i < W0 * rp; i += 64) T[i] = 0;
+  wave_fence();
+  match_rows(
+      a, U, d0, Xs, rows, v0,
+      [&](uint64_t i, uint32_t j) {
+        or64(&T[(uint64_t)(j >> 6) * rp + i], 1ull << (j & 63));
+        or64(&T[(uint64_t)(i >> 6) * rp + j], 1ull << (i & 63));
+      },
+      [&](uint64_t i, uint32_t wd, uint64_t bits) { or64(&T[(uint64_t)wd * rp + i], bits); });
+  wave_fence();
+  if (l < W0) {
+    uint64_t o = 0;
+    for (uint64_t x = 0; x < xq0; ++x) o |= T[(uint64_t)l * rp + d0 + x];
+    cm[l] = o;
+  }
+  wave_fence();
+  acc.build += 4 * (e0 - b0) + 4 * (d0 + x0) + 8 * W0 * rp;
+  G.v0 = v0;
+  G.d0 = (uint32_t)d0;
+  G.W0 = (uint32_t)W0;
+  G.xq0 = (uint32_t)xq0;
+  G.rp = (uint32_t)rp;
+  G.o0 = o0;
+  G.rfirst = a.rfirst[v0];
+  G.T = T;
+  G.cm = cm;
+  return true;
+}
+
+// Bits of `row` at the set positions of the mask (held one word per lane,
+// lanes < W0), compressed in order into NW words.  The mask is uniform, so
+// every lane runs the same loop.
+template <int NW>
+__device__ __forceinline__ void pext_row(const PkGroup &G, uint64_t mk, uint32_t row, bool act, uint64_t (&out)[NW]) {
 #pragma unroll
-  for (int b = 0; b < NX; ++b) {
-    uint64_t row[NW];
-#pragma unroll
-    for (int q = 0; q < NW; ++q) row[q] = (64u * b + l < xq) ? (uint64_t)sl[C::sXrow + (64u * b + l) * NW + q] : 0ull;
-#pragma unroll
-    for (int q = 0; q < NW; ++q) {
-      uint64_t mine = 0;
-      for (uint32_t j = 0; j < 64; ++j) {
-        const uint64_t col = __ballot((row[q] >> j) & 1);
-        if (l == j) mine = col;
+  for (int q = 0; q < NW; ++q) out[q] = 0;
+  uint32_t c = 0;
+  for (uint32_t q = 0; q < G.W0; ++q) {
+    uint64_t m = readlane64(mk, q);
+    if (m == 0) continue;
+    const uint64_t rw = act ? G.T[(uint64_t)q * G.rp + row] : 0ull;
+    for (; m; m &= m - 1, ++c) {
+      const uint64_t bit = (rw >> __builtin_ctzll(m)) & 1;
+      if constexpr (NW == 1) {
+        out[0] |= bit << c;
+      } else {
+        if (c < 64) out[0] |= bit << c;
+        else out[1] |= bit << (c - 64);
       }
-      if (64u * q + l < p) sl[C::sXcol + (64u * q + l) * NX + b] = mine;
     }
   }
-  if (l == 0) {
-    *w.m(mCanon, s) = canon;
-    *w.m(mKey, s) = sub.base_key;
-    *w.m(mPX, s) = p | (xq << 32);
-    *w.m(mBase, s) = (uint64_t)sub.base_size | ((uint64_t)sub.graph << 32);
-    *w.m(mV, s) = (uint64_t)sub.v0 | ((uint64_t)sub.v1 << 32);
-    *w.m(mNodes, s) = 0;
-    *w.m(mRefs, s) = 1;  // the root record pushed next
-    *w.m(mCount, s) = 0;
-    *w.m(mHash, s) = 0;
+}
+
+// Derives edge root `canon` = (v0, u_k) of group G into slot s (the whole
+// wave).  Returns as pk_build does.
+template <int NW, int NX>
+__device__ __forceinline__ uint32_t pk_derive(const PackWave<NW, NX> &w, const PkGroup &G, uint32_t s, uint64_t canon,
+                                              Acc &acc, uint64_t &cnt, uint64_t &hsh) {
+  using C = PackCfg<NW, NX>;
+  const KArgs &a = w.a;
+  const uint32_t l = lane();
+  const uint32_t k = (uint32_t)(canon - G.rfirst);
+  const uint32_t uk = a.col[G.o0 + k];
+  const uint32_t graph = a.root_graph ? a.root_graph[canon] : 0;
+  const uint64_t base_key = a.label_hash[G.v0] + a.label_hash[uk];
+  // Row k of U, split at k: P0 above it, the U part of X0 below it (one
+  // word per lane).
+  const uint32_t q = l;
+  const uint64_t ak = q < G.W0 ? G.T[(uint64_t)q * G.rp + k] : 0ull;
+  const uint32_t kq = k >> 6, kb = k & 63;
+  const uint64_t above = q > kq ? ~0ull : (q == kq ? (kb == 63 ? 0ull : ~0ull << (kb + 1)) : 0ull);
+  const uint64_t below = q < kq ? ~0ull : (q == kq ? low_mask(kb) : 0ull);
+  const uint64_t pk = ak & above, xu = ak & below;
+  const uint32_t p = __builtin_amdgcn_readlane(wave_scan_add_u32((uint32_t)__popcll(pk)), 63);
+  uint64_t words_read = G.W0;
+  if (p == 0) {
+    const bool xlow = (readlane64(q < G.W0 ? G.cm[q] : 0ull, kq) >> kb) & 1;
+    if (!xlow && __ballot(xu != 0) == 0) pk_emit_base(w, base_key, 2, graph, G.v0, uk, cnt, hsh);
+    acc.build += 8 * (words_read + 1);
+    return 0;
   }
-  if (a.root_graph)
-    for (uint32_t i = l; i < C::HS; i += 64) w.L.bhist[s * C::HS + i] = 0;
+  if (p > 64u * NW) {
+    if (l == 0) a.wide[atomicAdd(a.n_wide, 1ull)] = canon;
+    return 2;
+  }
+  // X0' rows, in vertex order: rows x of X0v' adjacent to u_k, then U rows
+  // j < k adjacent to u_k, each kept when it has a neighbour in P0.  Their
+  // row numbers go to a list in the slot's column area (rewritten below).
+  lds64 *sl = w.slot(s);
+  ldsu32 *xl = (ldsu32 *)(sl + C::sXcol);
+  uint32_t xq = 0;
+  for (uint32_t base = 0; base < G.xq0; base += 64) {
+    const uint32_t x = base + l;
+    const bool cand = x < G.xq0 && ((G.T[(uint64_t)kq * G.rp + G.d0 + x] >> kb) & 1);
+    bool keep = false;
+    for (uint32_t z = 0; z < G.W0; ++z) {  // uniform loop: the mask word is a readlane
+      const uint64_t mz = readlane64(pk, z);
+      if (cand && !keep) keep = (G.T[(uint64_t)z * G.rp + G.d0 + x] & mz) != 0;
+    }
+    const uint64_t m = __ballot(keep);
+    if (keep && xq + rank_below(m) < 64u * NX) xl[xq + rank_below(m)] = G.d0 + x;
+    xq += (uint32_t)__popcll(m);
+  }
+  words_read += (G.xq0 + 63) / 64 * 64;
+  const uint32_t nxu = __builtin_amdgcn_readlane(wave_scan_add_u32((uint32_t)__popcll(xu)), 63);
+  for (uint32_t base = 0; base < nxu; base += 64) {
+    // lane l takes the (base + l)-th U row below k adjacent to u_k
+    const uint32_t t = base + l;
+    uint32_t j = 0, before = 0;
+    for (uint32_t z = 0; z <= kq; ++z) {
+      const uint64_t wz = readlane64(xu, z);
+      const uint32_t cz = (uint32_t)__popcll(wz);
+      if (t >= before && t < before + cz) j = 64 * z + select64(wz, t - before);
+      before += cz;
+    }
+    bool keep = false;
+    for (uint32_t z = 0; z < G.W0; ++z) {
+      const uint64_t mz = readlane64(pk, z);
+      if (t < nxu && !keep) keep = (G.T[(uint64_t)z * G.rp + j] & mz) != 0;
+    }
+    const uint64_t m = __ballot(keep);
+    if (keep && xq + rank_below(m) < 64u * NX) xl[xq + rank_below(m)] = j;
+    xq += (uint32_t)__popcll(m);
+  }
+  words_read += (uint64_t)nxu * G.W0;
+  if (xq > 64u * NX) {
+    if (l == 0) a.wide[atomicAdd(a.n_wide, 1ull)] = canon;
+    return 2;
+  }
   wave_fence();
+  // Row numbers (in G) of the edge root's rows: P0 (the set bits of pk in
+  // order) and X0' (the list).
+  uint32_t prow_id[NW], xrow_id[NX];
+#pragma unroll
+  for (int r = 0; r < NW; ++r) {
+    const uint32_t t = 64u * r + l;
+    uint32_t j = 0, before = 0;
+    for (uint32_t z = 0; z < G.W0; ++z) {
+      const uint64_t wz = readlane64(pk, z);
+      const uint32_t cz = (uint32_t)__popcll(wz);
+      if (t >= before && t < before + cz) j = 64 * z + select64(wz, t - before);
+      before += cz;
+    }
+    prow_id[r] = j;
+  }
+#pragma unroll
+  for (int r = 0; r < NX; ++r) {
+    const uint32_t t = 64u * r + l;
+    xrow_id[r] = t < xq ? (uint32_t)xl[t] : 0u;
+  }
+  wave_fence();  // the list is read before the column area is rewritten
+  for (uint32_t i = l; i < C::sXcol; i += 64) sl[i] = 0;
+#pragma unroll
+  for (int r = 0; r < NW; ++r) {
+    const uint32_t t = 64u * r + l;
+    uint64_t o[NW];
+    pext_row<NW>(G, pk, prow_id[r], t < p, o);
+    if (t < p) {
+#pragma unroll
+      for (int z = 0; z < NW; ++z) sl[C::sProw + t * NW + z] = o[z];
+      const uint32_t u = a.col[G.o0 + prow_id[r]];
+      sl[C::sG + t] = a.label_hash[u];
+      if (a.list) a.pk_plist[((uint64_t)w.me * w.S + s) * kPackMaxP + t] = u;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NX; ++r) {
+    const uint32_t t = 64u * r + l;
+    uint64_t o[NW];
+    pext_row<NW>(G, pk, xrow_id[r], t < xq, o);
+    if (t < xq) {
+#pragma unroll
+      for (int z = 0; z < NW; ++z) sl[C::sXrow + t * NW + z] = o[z];
+    }
+  }
+  words_read += (uint64_t)(p + xq) * G.W0;
+  acc.build += 8 * words_read + 8ull * NW * (p + xq);
+  wave_fence();
+  pk_seal<NW, NX>(w, s, p, xq, canon, base_key, 2, graph, G.v0, uk);
   return 1;
 }
 
@@ -303,19 +554,37 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
   uint32_t busy = 0;                                 // occupied slots
   const uint32_t full = S >= 32 ? ~0u : ((1u << S) - 1);
   bool roots_left = true;
+  // Dispatch: runs of consecutive dispatch entries (edge roots of one v0,
+  // a.pk_gstart) or single entries; a run of two or more edge roots shares
+  // the vertex bitmap of its v0 (pk_group_build).
+  uint64_t gcur = 0, gend = 0;
+  bool grouped = false;
+  PkGroup G{};
   for (;;) {
     // Refill: build roots into free slots while fewer than 64 tasks wait.
     while (roots_left && pending < 64 && busy != full) {
-      unsigned long long i = 0;
-      if (l == 0) i = atomicAdd(a.next_root, 1ull);
-      i = bcast64(i);
-      if (i >= n_dispatch) {
-        roots_left = false;
-        break;
+      if (gcur == gend) {
+        unsigned long long gi = 0;
+        if (l == 0) gi = atomicAdd(a.next_root, 1ull);
+        gi = bcast64(gi);
+        if (a.pk_gstart ? gi >= a.pk_ngroups : gi >= n_dispatch) {
+          roots_left = false;
+          break;
+        }
+        gcur = a.pk_gstart ? a.pk_gstart[gi] : gi;
+        gend = a.pk_gstart ? a.pk_gstart[gi + 1] : gi + 1;
+        grouped = false;
+        if (gend - gcur >= 2 && a.root_mode == 2) {
+          const uint64_t c0 = a.perm ? a.perm[gcur] : (a.reverse ? n_dispatch - 1 - gcur : gcur);
+          grouped = pk_group_build(a, scratch, (uint32_t)(a.roots[c0] >> 32), G, acc);
+        }
       }
+      const uint64_t i = gcur++;
       const uint64_t canon = a.perm ? a.perm[i] : (a.reverse ? n_dispatch - 1 - i : i);
       const uint32_t s = (uint32_t)__builtin_ctzll((uint64_t)(~busy & full));
-      if (pk_build<NW, NX>(w, scratch, s, canon, acc, cnt, hsh) == 1) {
+      const uint32_t built = grouped ? pk_derive<NW, NX>(w, G, s, canon, acc, cnt, hsh)
+                                     : pk_build<NW, NX>(w, scratch, s, canon, acc, cnt, hsh);
+      if (built == 1) {
         if (l == 0) {
           uint64_t *rec = stk + (uint64_t)top * C::RS;
           for (uint32_t q = 0; q < C::RS; ++q) rec[q] = 0;
