@@ -837,7 +837,7 @@ hipError_t enqueue_run(const mce_plan *p, const KArgs &a, hipStream_t s) {
       b.n_dispatch_dev = p->d_ctr + kNWide1;
       b.next_root = p->d_ctr + kNext2;
     }
-    e = launch_pack(b, v, pw, s);
+    e = launch_pack(b, v, pw, (uint32_t)p->pack_wpc, s);
   }
   if (e != hipSuccess) return e;
   KArgs c = a;

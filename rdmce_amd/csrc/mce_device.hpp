@@ -158,8 +158,10 @@ constexpr uint64_t pack_lds_bytes(uint32_t nw, uint32_t nx, uint32_t slots, bool
 // waves_per_cu selects the kernel variant compiled for that occupancy
 // (<= 16: 4 waves per SIMD, <= 20: 5, <= 24: 6, else 8).
 hipError_t launch_mce(const KArgs &a, uint32_t waves, uint32_t waves_per_cu, hipStream_t stream);
-// Packed search: variant 0 = (NW, NX) = (1, 1), 1 = (2, 2).
-hipError_t launch_pack(const KArgs &a, int variant, uint32_t waves, hipStream_t stream);
+// Packed search: variant 0 = (NW, NX) = (1, 1), 1 = (2, 2); waves_per_cu
+// selects the kernel compiled for that occupancy (<= 16: 4 waves per SIMD,
+// <= 20: 5, <= 24: 6, else 8).
+hipError_t launch_pack(const KArgs &a, int variant, uint32_t waves, uint32_t waves_per_cu, hipStream_t stream);
 hipError_t launch_copy_probe(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
 // Zero-fills n 64-bit words with a kernel (safe inside captured hipGraphs,
 // unlike memset nodes).

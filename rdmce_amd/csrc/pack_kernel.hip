@@ -756,19 +756,35 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
           }
 #pragma unroll
           for (int q = 0; q < NW; ++q) {
-            for (uint64_t m = P[q]; m; m &= m - 1) {
-              const uint32_t i = 64u * q + (uint32_t)__builtin_ctzll(m);
-              uint64_t ri[NW];
-              uint32_t c = 0;
+            // two rows per pass, both loads in flight before either is used
+            for (uint64_t m = P[q]; m;) {
+              const uint64_t b0 = m & (~m + 1);
+              m ^= b0;
+              const uint64_t b1 = m & (~m + 1);
+              m ^= b1;
+              const uint32_t i0 = 64u * q + (uint32_t)__builtin_ctzll(b0);
+              const uint32_t i1 = b1 ? 64u * q + (uint32_t)__builtin_ctzll(b1) : i0;
+              uint64_t r0[NW], r1[NW];
+              uint32_t c0 = 0, c1 = 0;
 #pragma unroll
               for (int z = 0; z < NW; ++z) {
-                ri[z] = sl[C::sProw + i * NW + z];
-                c += (uint32_t)__popcll(P[z] & ~ri[z]);
+                r0[z] = sl[C::sProw + i0 * NW + z];
+                r1[z] = sl[C::sProw + i1 * NW + z];
               }
-              if (c == 1) {
-                mk[q] |= m & (~m + 1);
 #pragma unroll
-                for (int z = 0; z < NW; ++z) xacc[z] &= ri[z];
+              for (int z = 0; z < NW; ++z) {
+                c0 += (uint32_t)__popcll(P[z] & ~r0[z]);
+                c1 += (uint32_t)__popcll(P[z] & ~r1[z]);
+              }
+              if (c0 == 1) {
+                mk[q] |= b0;
+#pragma unroll
+                for (int z = 0; z < NW; ++z) xacc[z] &= r0[z];
+              }
+              if (b1 && c1 == 1) {
+                mk[q] |= b1;
+#pragma unroll
+                for (int z = 0; z < NW; ++z) xacc[z] &= r1[z];
               }
             }
           }
@@ -818,30 +834,38 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
         // pivot: (|P' \ N(u)|, row) argmin; rows of P0 first, then X0'
         uint32_t best = ~0u;
         bool zero = false;
+        // Rows two per pass (both loads in flight first); the minimum does
+        // not depend on the order rows are visited in.
+        auto scan = [&](uint64_t m, uint32_t rbase, uint32_t kbase) {
+          while (m && !zero) {
+            const uint64_t b0 = m & (~m + 1);
+            m ^= b0;
+            const uint64_t b1 = m & (~m + 1);
+            m ^= b1;
+            const uint32_t u0 = (uint32_t)__builtin_ctzll(b0);
+            const uint32_t u1 = b1 ? (uint32_t)__builtin_ctzll(b1) : u0;
+            uint64_t r0[NW], r1[NW];
 #pragma unroll
-        for (int q = 0; q < NW; ++q) {
-          for (uint64_t m = P[q] | XP[q]; m && !zero; m &= m - 1) {
-            const uint32_t u = 64u * q + (uint32_t)__builtin_ctzll(m);
-            uint32_t c = 0;
+            for (int z = 0; z < NW; ++z) {
+              r0[z] = sl[rbase + u0 * NW + z];
+              r1[z] = sl[rbase + u1 * NW + z];
+            }
+            uint32_t c0 = 0, c1 = 0;
 #pragma unroll
-            for (int z = 0; z < NW; ++z) c += (uint32_t)__popcll(P[z] & ~(uint64_t)sl[C::sProw + u * NW + z]);
-            const uint32_t kx = (c << 16) | u;
-            best = kx < best ? kx : best;
-            zero = c == 0;
+            for (int z = 0; z < NW; ++z) {
+              c0 += (uint32_t)__popcll(P[z] & ~r0[z]);
+              c1 += (uint32_t)__popcll(P[z] & ~r1[z]);
+            }
+            const uint32_t k0 = (c0 << 16) | (kbase + u0), k1 = (c1 << 16) | (kbase + u1);
+            best = k0 < best ? k0 : best;
+            best = k1 < best ? k1 : best;
+            zero = c0 == 0 || c1 == 0;
           }
-        }
+        };
 #pragma unroll
-        for (int j = 0; j < NX; ++j) {
-          for (uint64_t m = XX[j]; m && !zero; m &= m - 1) {
-            const uint32_t x = 64u * j + (uint32_t)__builtin_ctzll(m);
-            uint32_t c = 0;
+        for (int q = 0; q < NW; ++q) scan(P[q] | XP[q], C::sProw + 64u * q * NW, 64u * q);
 #pragma unroll
-            for (int z = 0; z < NW; ++z) c += (uint32_t)__popcll(P[z] & ~(uint64_t)sl[C::sXrow + x * NW + z]);
-            const uint32_t kx = (c << 16) | (p + x);
-            best = kx < best ? kx : best;
-            zero = c == 0;
-          }
-        }
+        for (int j = 0; j < NX; ++j) scan(XX[j], C::sXrow + 64u * j * NW, p + 64u * j);
         if (!zero) {  // else an excluded row covers P': no maximal clique below
           words += Wr;
           const uint32_t piv = best & 0xFFFF;
@@ -906,22 +930,32 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
     if (w.L.hist[i]) atomicAdd(&a.hist[i], (unsigned long long)w.L.hist[i]);
 }
 
-template <int NW, int NX>
-__global__ __launch_bounds__(64) void pack_kernel(KArgs a) {
+// One kernel per variant and occupancy target (WPE waves per SIMD: the
+// register budget 512 / WPE VGPRs).
+template <int NW, int NX, int WPE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void pack_kernel(KArgs a) {
   pack_main<NW, NX>(a);
+}
+
+template <int NW, int NX>
+hipError_t launch_variant(const KArgs &a, uint32_t waves, uint32_t wpc, size_t lds, hipStream_t stream) {
+  if (wpc <= 16)
+    hipLaunchKernelGGL((pack_kernel<NW, NX, 4>), dim3(waves), dim3(64), lds, stream, a);
+  else if (wpc <= 20)
+    hipLaunchKernelGGL((pack_kernel<NW, NX, 5>), dim3(waves), dim3(64), lds, stream, a);
+  else if (wpc <= 24)
+    hipLaunchKernelGGL((pack_kernel<NW, NX, 6>), dim3(waves), dim3(64), lds, stream, a);
+  else
+    hipLaunchKernelGGL((pack_kernel<NW, NX, 8>), dim3(waves), dim3(64), lds, stream, a);
+  return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_pack(const KArgs &a, int variant, uint32_t waves, hipStream_t stream) {
+hipError_t launch_pack(const KArgs &a, int variant, uint32_t waves, uint32_t waves_per_cu, hipStream_t stream) {
   const bool batch = a.root_graph != nullptr;
-  if (variant == 0)
-    hipLaunchKernelGGL((pack_kernel<1, 1>), dim3(waves), dim3(64), pack_lds_bytes(1, 1, a.pk_slots, batch), stream,
-                       a);
-  else
-    hipLaunchKernelGGL((pack_kernel<2, 2>), dim3(waves), dim3(64), pack_lds_bytes(2, 2, a.pk_slots, batch), stream,
-                       a);
-  return hipGetLastError();
+  if (variant == 0) return launch_variant<1, 1>(a, waves, waves_per_cu, pack_lds_bytes(1, 1, a.pk_slots, batch), stream);
+  return launch_variant<2, 2>(a, waves, waves_per_cu, pack_lds_bytes(2, 2, a.pk_slots, batch), stream);
 }
 
 }  // namespace rdmce
