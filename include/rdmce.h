@@ -135,7 +135,12 @@ typedef struct mce_options {
                                 path).  Never changes a result. */
   uint32_t pack_slots;       /* subproblems resident per wavefront in the packed search (1..8; 0 = 4) */
   uint32_t pack_waves_per_cu;/* wavefronts per CU of the packed search (1..32; 0 = 16) */
-  uint64_t reserved[4];
+  const uint64_t *runs;      /* optional, packed search over a custom edge-root list: run r is
+                                roots[runs[r] .. runs[r + 1]), all with one first vertex, whose
+                                bitmap the run builds once; runs[0] = 0, runs[n_runs] = n_roots.
+                                NULL: maximal runs of consecutive entries sharing a first vertex */
+  uint64_t n_runs;
+  uint64_t reserved[2];
 } mce_options;
 
 void mce_options_init(mce_options *opt);
@@ -225,15 +230,21 @@ typedef struct mce_tune_result {
   int32_t trials;
   double best_ms;            /* kernel time of the winning setting on the sample */
   double base_ms;            /* kernel time of the default setting on the sample */
-  uint64_t reserved[4];
+  uint32_t pack_slots;       /* packed search: subproblems resident per wavefront (the LDS tile width) */
+  uint32_t pack_waves_per_cu;/* packed search: wavefronts per CU */
+  uint64_t reserved[3];
 } mce_tune_result;
 
 /* Times the plan's kernels on a deterministic sample of its roots (every
- * `stride`-th dispatched root; 1 = all, the faithful choice: a sparse sample
- * changes the load balance the settings are about) for a coordinate search:
- * waves per CU in {8, 12, 16, 20, 24, 28, 32} with the LDS tier on, then the sharing
- * bound in {16, 24, 48, off} at the best occupancy, then waves per CU again
- * when the bound changed, then the LDS tier off.
+ * `stride`-th run of edge roots sharing a first vertex, whole - or every
+ * `stride`-th root without runs; 1 = all, the faithful choice: a sparse
+ * sample changes the load balance the settings are about) for a coordinate
+ * search.  One-root-per-wavefront plans: waves per CU in {8, 12, 16, 20, 24,
+ * 28, 32} with the LDS tier on, then the sharing bound in {16, 24, 48, off}
+ * at the best occupancy, then waves per CU again when the bound changed, then
+ * the LDS tier off.  Packed plans (mce_options.pack): subproblems per
+ * wavefront in {1, 2, 4, 8}, then packed wavefronts per CU in {12, 16, 20,
+ * 24}, then the sharing bound of the wide roots' kernel in {24, off}.
  * Checks that every setting gives the same clique count and hash, and keeps
  * the fastest for later mce_plan_run calls. */
 mce_status mce_plan_autotune(mce_plan *p, uint64_t stride, mce_tune_result *out);

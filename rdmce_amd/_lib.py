@@ -49,7 +49,9 @@ class MceOptions(Structure):
         ("pack", c_int32),
         ("pack_slots", c_uint32),
         ("pack_waves_per_cu", c_uint32),
-        ("reserved", c_uint64 * 4),
+        ("runs", POINTER(c_uint64)),
+        ("n_runs", c_uint64),
+        ("reserved", c_uint64 * 2),
     ]
 
 
@@ -78,7 +80,9 @@ class MceTuneResult(Structure):
         ("trials", c_int32),
         ("best_ms", c_double),
         ("base_ms", c_double),
-        ("reserved", c_uint64 * 4),
+        ("pack_slots", c_uint32),
+        ("pack_waves_per_cu", c_uint32),
+        ("reserved", c_uint64 * 3),
     ]
 
 

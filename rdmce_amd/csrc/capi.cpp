@@ -431,6 +431,7 @@ struct mce_plan {
   uint64_t *d_rfirst = nullptr;
   uint64_t *d_gstart = nullptr;      // runs of the plan's dispatch list sharing their first vertex
   uint64_t n_groups = 0;
+  std::vector<uint64_t> h_runs;      // host copy of d_gstart
   // captured run (hipGraph)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -633,11 +634,26 @@ mce_status plan_create_impl(const PlanSpec &spec, const mce_options &opt, mce_pl
     PTRY(hipMemcpy(p->d_rfirst, p->h_rfirst.data(), (g.n + 1) * 8, hipMemcpyHostToDevice));
     const uint64_t nd = p->n_dispatch;
     const uint64_t *cr = opt.schedule == MCE_SCHED_CUSTOM ? opt.roots : nullptr;
-    std::vector<uint64_t> runs =
-        dispatch_runs(p->h_rfirst, nd, [&](uint64_t i) { return cr ? cr[i] : nd - 1 - i; });
+    std::vector<uint64_t> runs;
+    if (opt.runs && cr) {
+      // caller runs: boundaries ascending from 0 to n_roots, one first vertex each
+      if (opt.runs[0] != 0 || opt.runs[opt.n_runs] != nd)
+        return bail(fail(MCE_ERR_ARG, "runs must start at 0 and end at the root count"));
+      for (uint64_t r = 0; r < opt.n_runs; ++r) {
+        if (opt.runs[r] >= opt.runs[r + 1]) return bail(fail(MCE_ERR_ARG, "runs must be strictly ascending"));
+        const uint64_t lo = p->h_rfirst[(uint64_t)(roots[cr[opt.runs[r]]] >> 32)];
+        const uint64_t hi = p->h_rfirst[(uint64_t)(roots[cr[opt.runs[r]]] >> 32) + 1];
+        for (uint64_t i = opt.runs[r]; i < opt.runs[r + 1]; ++i)
+          if (cr[i] < lo || cr[i] >= hi) return bail(fail(MCE_ERR_ARG, "a run mixes first vertices"));
+      }
+      runs.assign(opt.runs, opt.runs + opt.n_runs + 1);
+    } else {
+      runs = dispatch_runs(p->h_rfirst, nd, [&](uint64_t i) { return cr ? cr[i] : nd - 1 - i; });
+    }
     p->n_groups = runs.size() - 1;
     PTRY(dalloc(&p->d_gstart, runs.size()));
     PTRY(hipMemcpy(p->d_gstart, runs.data(), runs.size() * 8, hipMemcpyHostToDevice));
+    p->h_runs = std::move(runs);
   }
   if (opt.eager) {
     PTRY(dalloc(&p->d_eager, p->n_dispatch));
@@ -829,7 +845,9 @@ hipError_t enqueue_run(const mce_plan *p, const KArgs &a, hipStream_t s) {
     b.pk_plist = p->d_pk_plist;
     b.wide = p->d_wide[v];
     b.n_wide = p->d_ctr + (v == 0 ? kNWide1 : kNWide2);
-    b.eager = nullptr;
+    b.heavy = p->d_wide[1];
+    b.n_heavy = p->d_ctr + kNWide2;
+    if (v == 1) b.eager = nullptr;
     if (v == 1) {
       b.pk_gstart = nullptr;
       b.perm = p->d_wide[0];
@@ -1028,22 +1046,29 @@ mce_status plan_run_impl(mce_plan *p, mce_result **out) {
 mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
   HIP_TRY(hipSetDevice(p->device));
   if (stride == 0) stride = 1;
-  // Sample: every stride-th dispatched root.
+  // Sample: every stride-th run of edge roots sharing a first vertex, whole
+  // (runs share a build in the packed search), or every stride-th root.
   std::vector<uint64_t> disp(p->n_dispatch);
   if (p->d_perm) HIP_TRY(hipMemcpy(disp.data(), p->d_perm, p->n_dispatch * 8, hipMemcpyDeviceToHost));
   else
     for (uint64_t i = 0; i < p->n_dispatch; ++i) disp[i] = p->n_dispatch - 1 - i;
-  std::vector<uint64_t> sample;
-  for (uint64_t i = 0; i < disp.size(); i += stride) sample.push_back(disp[i]);
+  std::vector<uint64_t> sample, sruns;
+  if (!p->h_runs.empty()) {
+    sruns.push_back(0);
+    for (uint64_t r = 0; r + 1 < p->h_runs.size(); r += stride) {
+      for (uint64_t i = p->h_runs[r]; i < p->h_runs[r + 1]; ++i) sample.push_back(disp[i]);
+      sruns.push_back(sample.size());
+    }
+  } else {
+    for (uint64_t i = 0; i < disp.size(); i += stride) sample.push_back(disp[i]);
+  }
   uint64_t *d_sample = nullptr, *d_sruns = nullptr;
   HIP_TRY(dalloc(&d_sample, sample.size()));
   if (!sample.empty()) HIP_TRY(hipMemcpy(d_sample, sample.data(), sample.size() * 8, hipMemcpyHostToDevice));
-  uint64_t n_sruns = 0;
-  if (!p->h_rfirst.empty()) {
-    std::vector<uint64_t> runs = dispatch_runs(p->h_rfirst, sample.size(), [&](uint64_t i) { return sample[i]; });
-    n_sruns = runs.size() - 1;
-    HIP_TRY(dalloc(&d_sruns, runs.size()));
-    HIP_TRY(hipMemcpy(d_sruns, runs.data(), runs.size() * 8, hipMemcpyHostToDevice));
+  const uint64_t n_sruns = sruns.empty() ? 0 : sruns.size() - 1;
+  if (n_sruns) {
+    HIP_TRY(dalloc(&d_sruns, sruns.size()));
+    HIP_TRY(hipMemcpy(d_sruns, sruns.data(), sruns.size() * 8, hipMemcpyHostToDevice));
   }
   // Waves per CU first (LDS tier on), then the sharing bound at the best
   // occupancy, then waves per CU again when the bound changed, then the LDS
@@ -1063,6 +1088,7 @@ mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
     a.reverse = 0;
     a.pk_gstart = d_sruns;
     a.pk_ngroups = n_sruns;
+    a.eager = nullptr;
     RunOut ro;
     mce_status s = run_core(p, a, false, ro);
     if (s) return s;
@@ -1082,6 +1108,63 @@ mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
   st = trial(base_wpc, base_tau, base_lds, ms);
   if (!st) st = trial(base_wpc, base_tau, base_lds, base_ms);
   if (!st) best = base_ms;
+  if (p->pack) {
+    // Packed plans: subproblems per wave (the LDS tile width), packed waves
+    // per CU, then the sharing bound of the wide roots' kernel.
+    const uint32_t base_slots = p->pack_slots;
+    const int base_pwpc = p->pack_wpc;
+    uint32_t best_slots = base_slots;
+    int best_pwpc = base_pwpc;
+    for (uint32_t sl : {1u, 2u, 4u, 8u}) {
+      if (st || sl == base_slots) continue;
+      p->pack_slots = sl;
+      st = trial(base_wpc, base_tau, base_lds, ms);
+      if (!st && ms < best) {
+        best = ms;
+        best_slots = sl;
+      }
+    }
+    p->pack_slots = best_slots;
+    for (int pw : {12, 16, 20, 24}) {
+      if (st || pw == base_pwpc) continue;
+      p->pack_wpc = pw;
+      st = trial(base_wpc, base_tau, base_lds, ms);
+      if (!st && ms < best) {
+        best = ms;
+        best_pwpc = pw;
+      }
+    }
+    p->pack_wpc = best_pwpc;
+    for (int tau : {24, -1}) {
+      if (st || tau == base_tau) continue;
+      st = trial(base_wpc, tau, base_lds, ms);
+      if (!st && ms < best) {
+        best = ms;
+        best_tau = tau;
+      }
+    }
+    (void)hipFree(d_sample);
+    if (d_sruns) (void)hipFree(d_sruns);
+    if (st) {
+      p->pack_slots = base_slots;
+      p->pack_wpc = base_pwpc;
+      set_config(p, base_wpc, base_tau, base_lds);
+      return st;
+    }
+    set_config(p, base_wpc, best_tau, base_lds);
+    if (out) {
+      std::memset(out, 0, sizeof(*out));
+      out->waves_per_cu = base_wpc;
+      out->share_bound = best_tau;
+      out->lds_tier = base_lds;
+      out->trials = trials;
+      out->best_ms = best;
+      out->base_ms = base_ms;
+      out->pack_slots = p->pack_slots;
+      out->pack_waves_per_cu = (uint32_t)p->pack_wpc;
+    }
+    return MCE_OK;
+  }
   for (int wpc : wpcs) {
     if ((uint64_t)p->cus * wpc > p->max_waves || st || (wpc == base_wpc && base_lds == 1)) continue;
     st = trial(wpc, base_tau, 1, ms);
@@ -1136,6 +1219,8 @@ mce_status autotune_impl(mce_plan *p, uint64_t stride, mce_tune_result *out) {
     out->trials = trials;
     out->best_ms = best;
     out->base_ms = base_ms;
+    out->pack_slots = p->pack_slots;
+    out->pack_waves_per_cu = (uint32_t)p->pack_wpc;
   }
   return MCE_OK;
 }

@@ -129,7 +129,15 @@ struct Acc {
   unsigned long long hash;
   unsigned long long words;
   unsigned long long build;
+  unsigned long long scan;  // per lane: adjacency entries of the bitmap rows' runs
 };
+
+// Sum over the wave of a per-lane 64-bit value (uniform result).
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+  uint64_t s = 0;
+  for (uint32_t k = 0; k < 64; ++k) s += readlane64(v, k);
+  return s;
+}
 
 __device__ __forceinline__ unsigned int ld_relaxed(unsigned int *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -312,12 +320,15 @@ __device__ __forceinline__ uint64_t compact_x(const KArgs &a, const uint32_t *P,
 // adjacency run against P; with fewer than 64 rows, g = 2 or 4 lanes share a
 // row, each matching one share of its P range (a row's cost is a chain of
 // dependent loads into adjacency lists; splitting the range shortens the
-// chain), so callees must set bits with atomic ORs.
+// chain), so callees must set bits with atomic ORs.  Returns this lane's
+// share of the adjacency entries in the runs matched (each row's run counted
+// once, by its first lane).
 template <class PP, class XW>
-__device__ __forceinline__ void match_rows(const KArgs &a, const uint32_t *P, uint64_t p, const uint32_t *X,
-                                           uint64_t rows, uint32_t above, PP pp, XW xw) {
+__device__ __forceinline__ uint64_t match_rows(const KArgs &a, const uint32_t *P, uint64_t p, const uint32_t *X,
+                                               uint64_t rows, uint32_t above, PP pp, XW xw) {
   const uint32_t l = lane();
   const uint32_t *col = a.col;
+  uint64_t scanned = 0;
   const uint32_t g = rows <= 16 ? 4u : (rows <= 32 ? 2u : 1u);
   const uint32_t gs = g == 4 ? 2u : (g == 2 ? 1u : 0u);
   for (uint64_t t = l; t < rows << gs; t += 64) {
@@ -332,6 +343,7 @@ __device__ __forceinline__ void match_rows(const KArgs &a, const uint32_t *P, ui
       const uint32_t u = P[i];
       const uint64_t e = a.rowoff[u + 1];
       const uint64_t rb = sh ? lower_bound(col, a.outoff[u], e, P[jlo]) : a.outoff[u];
+      if (!sh) scanned += e - rb;
       match_run(col, rb, e, P, jlo, jhi, [&](uint32_t j) {
         pp(i, j);
         return false;
@@ -340,6 +352,7 @@ __device__ __forceinline__ void match_rows(const KArgs &a, const uint32_t *P, ui
       const uint32_t u = X[i - p];
       const uint64_t e = a.rowoff[u + 1];
       const uint64_t rb = lower_bound(col, a.outoff[u], e, sh ? P[jlo] : above + 1);
+      if (!sh) scanned += e - rb;
       uint32_t curw = ~0u;
       uint64_t accw = 0;
       match_run(col, rb, e, P, jlo, jhi, [&](uint32_t b) {
@@ -355,6 +368,7 @@ __device__ __forceinline__ void match_rows(const KArgs &a, const uint32_t *P, ui
       if (curw != ~0u) xw(i, curw, accw);
     }
   }
+  return scanned;
 }
 
 }  // namespace

@@ -134,6 +134,11 @@ struct KArgs {
   const uint64_t *pk_gstart;
   uint64_t pk_ngroups;
   const uint64_t *rfirst;
+  // Packed search: dispatch entries flagged in `eager` (roots predicted
+  // heavy) are appended here for the one-root-per-wave kernel, which splits
+  // them across waves.
+  uint64_t *heavy;
+  unsigned long long *n_heavy;
 };
 
 // Packed search variants: |P0| <= 64 NW and |X0'| <= 64 NX.

@@ -1037,7 +1037,7 @@ __device__ __forceinline__ uint64_t solve_root(const KArgs &a, uint8_t *slot, ui
     for (uint64_t i = l; i < nbw; i += 64) T[i] = 0;
     for (uint64_t i = l; i < p; i += 64) G[i] = a.label_hash[P[i]];
     wave_fence();
-    match_rows(a, P, p, X, rows, above,
+    acc.scan += match_rows(a, P, p, X, rows, above,
                [&](uint64_t i, uint32_t j) {
                  or64(&T[(uint64_t)(j >> 6) * rows_pad + i], 1ull << (j & 63));
                  or64(&T[(uint64_t)(i >> 6) * rows_pad + j], 1ull << (i & 63));
@@ -1211,7 +1211,7 @@ __device__ __forceinline__ void mce_main(const KArgs &a) {
   const uint32_t me = blockIdx.x;
   uint8_t *slot = a.slots + (uint64_t)me * a.slot_bytes;
   uint64_t *stk = (uint64_t *)(a.stacks + (uint64_t)me * a.stack_bytes);
-  Acc acc{0, 0, 0, 0};
+  Acc acc{0, 0, 0, 0, 0};
   unsigned long long nodes_total = 0;
   // A wave is working (counted in `active`) until it finds neither a root
   // nor a task; it is then looking for work (counted in the idle copies)
@@ -1281,6 +1281,7 @@ __device__ __forceinline__ void mce_main(const KArgs &a) {
     backoff = backoff < 64 ? backoff * 2 : 64;
   }
   __syncthreads();
+  acc.build += 4 * wave_sum64(acc.scan);
   if (l == 0) {
     if (acc.count) atomicAdd(a.count, acc.count);
     if (acc.hash) atomicAdd(a.hash, acc.hash);

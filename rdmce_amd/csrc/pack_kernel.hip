@@ -72,13 +72,6 @@ __device__ __forceinline__ uint32_t select64(uint64_t w, uint32_t k) {
   return pos;
 }
 
-// Sum over the wave of a per-lane 64-bit value (uniform result).
-__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
-  uint64_t s = 0;
-  for (uint32_t k = 0; k < 64; ++k) s += readlane64(v, k);
-  return s;
-}
-
 // The per-wave LDS areas (word offsets from lds_dyn).
 struct PackLds {
   ldsu64 *hist;     // kLdsHist clique sizes (outside batches)
@@ -264,7 +257,7 @@ __device__ __forceinline__ uint32_t pk_build(const PackWave<NW, NX> &w, uint8_t 
     for (uint32_t i = l; i < p; i += 64) pl[i] = P[i];
   }
   wave_fence();
-  match_rows(
+  acc.scan += match_rows(
       a, P, p, X, rows, above,
       [&](uint64_t i, uint32_t j) {
         or64_lds(sl + C::sProw + i * NW + (j >> 6), 1ull << (j & 63));
@@ -325,7 +318,7 @@ __device__ __forceinline__ bool pk_group_build(const KArgs &a, uint8_t *scratch,
   uint64_t *cm = (uint64_t *)(scratch + cm_off);
   for (uint64_t i = l; i < W0 * rp; i += 64) T[i] = 0;
   wave_fence();
-  match_rows(
+  acc.scan += match_rows(
       a, U, d0, Xs, rows, v0,
       [&](uint64_t i, uint32_t j) {
         or64(&T[(uint64_t)(j >> 6) * rp + i], 1ull << (j & 63));
@@ -547,7 +540,7 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
   uint8_t *scratch = a.slots + (uint64_t)w.me * a.slot_bytes;
   uint64_t *stk = a.pk_stack + (uint64_t)w.me * a.pk_cap * C::RS;
   ldsu32 *map32 = (ldsu32 *)w.L.map;
-  Acc acc{0, 0, 0, 0};
+  Acc acc{0, 0, 0, 0, 0};
   uint64_t cnt = 0, hsh = 0, words = 0, nodes = 0;  // per lane
   uint32_t top = 0;                                  // records in the stack
   uint64_t pending = 0;                              // candidates (tasks) in the stack
@@ -581,6 +574,10 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
       }
       const uint64_t i = gcur++;
       const uint64_t canon = a.perm ? a.perm[i] : (a.reverse ? n_dispatch - 1 - i : i);
+      if (a.eager && a.eager[i]) {  // split before launch: to the one-root-per-wave kernel
+        if (l == 0) a.heavy[atomicAdd(a.n_heavy, 1ull)] = canon;
+        continue;
+      }
       const uint32_t s = (uint32_t)__builtin_ctzll((uint64_t)(~busy & full));
       const uint32_t built = grouped ? pk_derive<NW, NX>(w, G, s, canon, acc, cnt, hsh)
                                      : pk_build<NW, NX>(w, scratch, s, canon, acc, cnt, hsh);
@@ -918,6 +915,7 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
 
   // Exit: the wave's sums.
   const uint64_t wc = wave_sum64(cnt), wh = wave_sum64(hsh), ww = wave_sum64(words), wn = wave_sum64(nodes);
+  acc.build += 4 * wave_sum64(acc.scan);
   if (l == 0) {
     if (wc) atomicAdd(a.count, (unsigned long long)wc);
     if (wh) atomicAdd(a.hash, (unsigned long long)wh);

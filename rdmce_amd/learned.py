@@ -131,6 +131,7 @@ class Schedule:
     order: np.ndarray      # canonical root ids in dispatch order
     eager: np.ndarray      # uint8 per dispatch position
     learned: bool          # False: fell back to the plain order
+    runs: np.ndarray | None = None  # packed search: run boundaries in `order` (one first vertex per run)
 
 
 def plain_schedule(root_ids) -> Schedule:
@@ -168,6 +169,69 @@ def learned_schedule(pred, root_ids, eager_top: int = 0, heavy_fraction: float =
     eager = np.zeros(order.size, dtype=np.uint8)
     eager[:min(eager_top, order.size)] = 1
     return Schedule(order, eager, True)
+
+
+def learned_split(pred, root_ids, heavy_fraction: float) -> Schedule:
+    """Schedule for the packed search: the roots stay in the plain order
+    (runs of edge roots of one first vertex share that vertex's bitmap), and
+    the predicted-heaviest `heavy_fraction` of them are flagged, which sends
+    them to the one-root-per-wave kernel that splits them across waves - a
+    packed wave searches a root alone, so one heavy root would hold it to the
+    end of the launch.  Falls back to the plain order (no flags) when a
+    prediction is not finite."""
+    base = np.sort(np.asarray(root_ids, dtype=np.uint64))[::-1].copy()
+    p = np.asarray(pred, dtype=np.float64)[base.astype(np.int64)]
+    eager = np.zeros(base.size, dtype=np.uint8)
+    if p.size and not np.all(np.isfinite(p)):
+        return Schedule(base, eager, False)
+    k = int(round(min(max(heavy_fraction, 0.0), 1.0) * base.size))
+    if k:
+        eager[np.argsort(-p, kind="stable")[:k]] = 1
+    return Schedule(base, eager, True)
+
+
+def learned_group_schedule(pred, root_ids, first_vertex, waves: int, chunk_share: float = 0.25,
+                           route_share: float = 1.0) -> Schedule:
+    """Schedule for the packed search from per-root predicted costs.
+
+    The roots are taken in runs that share their first vertex v0 (the packed
+    search builds v0's bitmap once per run; `first_vertex` gives v0 for
+    every canonical root id).  With c = the predicted cost of a root and
+    S = sum(c) / waves (one wave's share of the whole run):
+      * a run predicted to cost more than chunk_share * S is split into
+        chunks of about that cost, which different waves take;
+      * runs and chunks are dispatched in decreasing predicted cost, so the
+        ones that could end last start first;
+      * a root predicted to cost more than route_share * S is flagged: the
+        packed search hands it to the one-root-per-wave kernel, which splits
+        it across waves (a packed wave searches a root alone).
+    Falls back to the plain order when a prediction is not finite."""
+    base = np.sort(np.asarray(root_ids, dtype=np.uint64))[::-1].copy()
+    logc = np.asarray(pred, dtype=np.float64)[base.astype(np.int64)]
+    if logc.size == 0:
+        return Schedule(base, np.zeros(0, dtype=np.uint8), False)
+    if not np.all(np.isfinite(logc)):
+        return plain_schedule(root_ids)
+    c = np.expm1(np.clip(logc, 0.0, 60.0))
+    share = c.sum() / max(int(waves), 1)
+    v0 = np.asarray(first_vertex)[base.astype(np.int64)]
+    new_run = np.empty(base.size, dtype=bool)
+    new_run[0] = True
+    new_run[1:] = v0[1:] != v0[:-1]
+    run = np.cumsum(new_run) - 1
+    cum = np.cumsum(c)
+    run_start_cum = (cum - c)[new_run][run]  # cost before the run's first root
+    chunk_cost = max(chunk_share * share, 1e-30)
+    chunk_in_run = np.floor((cum - c - run_start_cum) / chunk_cost).astype(np.int64)
+    piece = np.cumsum(new_run | np.r_[False, chunk_in_run[1:] != chunk_in_run[:-1]]) - 1
+    piece_cost = np.bincount(piece, weights=c)
+    rank = np.argsort(np.argsort(-piece_cost, kind="stable"), kind="stable")
+    perm = np.lexsort((np.arange(base.size), rank[piece]))
+    order = base[perm]
+    pieces = piece[perm]
+    runs = np.r_[0, np.flatnonzero(pieces[1:] != pieces[:-1]) + 1, order.size].astype(np.uint64)
+    eager = (c[perm] > route_share * share).astype(np.uint8)
+    return Schedule(order, eager, True, runs)
 
 
 def training_set(features, root_cycles, root_ids, max_samples: int = 1 << 20, seed: int = 0):

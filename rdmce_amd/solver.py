@@ -52,7 +52,7 @@ def make_options(device: int = 0, root_mode: int = _lib.ROOT_EDGE, waves_per_cu:
                  want_list: bool = False, want_root_stats: bool = False, list_capacity: int = 0,
                  slot_bytes: int = 0, root_order=None, lds_tier: int = 1, use_graph: bool = False,
                  lds_bytes: int = 0, eager=None, word_path: bool = True, pack: bool = False, pack_slots: int = 0,
-                 pack_waves_per_cu: int = 0) -> tuple[MceOptions, object]:
+                 pack_waves_per_cu: int = 0, runs=None) -> tuple[MceOptions, object]:
     o = MceOptions()
     lib().mce_options_init(byref(o))
     o.device = device
@@ -77,6 +77,11 @@ def make_options(device: int = 0, root_mode: int = _lib.ROOT_EDGE, waves_per_cu:
         o.schedule = _lib.SCHED_CUSTOM
         o.roots = ro.ctypes.data_as(POINTER(c_uint64))
         o.n_roots = ro.size
+    if runs is not None:
+        ru = np.ascontiguousarray(runs, dtype=np.uint64)
+        keep.append(ru)
+        o.runs = ru.ctypes.data_as(POINTER(c_uint64))
+        o.n_runs = max(ru.size - 1, 0)
     if eager is not None:
         ea = np.ascontiguousarray(eager, dtype=np.uint8)
         keep.append(ea)
@@ -150,7 +155,8 @@ class Plan:
 
 def _tune_dict(t) -> dict:
     return {"waves_per_cu": t.waves_per_cu, "share_bound": t.share_bound, "lds_tier": t.lds_tier,
-            "trials": t.trials, "best_ms": t.best_ms, "base_ms": t.base_ms}
+            "trials": t.trials, "best_ms": t.best_ms, "base_ms": t.base_ms, "pack_slots": t.pack_slots,
+            "pack_waves_per_cu": t.pack_waves_per_cu}
 
 
 @dataclasses.dataclass
