@@ -270,6 +270,8 @@ def main():
     ap.add_argument("--route-share", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also launch a 1 GiB streaming copy kernel (traffic counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -338,6 +340,13 @@ def main():
     plan.free()
     if not same(abl_outs[0], plain):
         raise SystemExit("batched plain-order results differ from the plain path")
+
+    if args.copy_probe:
+        import ctypes
+        from rdmce_amd._lib import check, lib
+        ms = ctypes.c_double()
+        check(lib().mce_copy_probe(0, 1 << 30, 4, ctypes.byref(ms)))
+        log(f"copy probe 1 GiB: {ms.value:.3f} ms")
 
     value = first.count * args.steps / wall
     line = {

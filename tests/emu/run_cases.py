@@ -144,6 +144,49 @@ def cases():
         return None
     yield "plan graph replay", replay
 
+    # packed search (pack_kernel.hip): one tree node per lane, subproblems
+    # resident in LDS slots, edge roots derived from shared vertex bitmaps
+    yield "pack zachary list", lambda: same(R.solve(g, waves_per_cu=wpc, pack=True, pack_waves_per_cu=2,
+                                                    want_list=True, want_root_stats=True),
+                                            oracle_for(g, list_cap=1 << 16), lists=True)
+    for n, p, seed in [(60, 0.5, 1), (45, 0.8, 3), (150, 0.45, 8)]:
+        rg = graph(random_graph(n, p, seed))
+        ref = oracle_for(rg)
+        for slots in (1, 4):
+            yield (f"pack G({n},{p}) slots={slots}",
+                   lambda rg=rg, ref=ref, slots=slots: same(
+                       R.solve(rg, waves_per_cu=wpc, pack=True, pack_slots=slots, pack_waves_per_cu=2,
+                               want_root_stats=True), ref))
+    yield "pack vertex two-word", lambda: same(R.solve(tg, waves_per_cu=wpc, root_mode=R.ROOT_VERTEX, pack=True,
+                                                       pack_waves_per_cu=2, want_root_stats=True), tref)
+
+    def pack_routed_runs():
+        # custom order, explicit runs split mid-vertex, every 5th root routed
+        # to the one-root-per-wave kernel
+        rg = graph(random_graph(120, 0.45, 9))
+        ref = oracle_for(rg)
+        m = rg.GetNumEdges()
+        order = np.arange(m, dtype=np.uint64)[::-1].copy()
+        v0 = (rg.roots(R.ROOT_EDGE) >> np.uint64(32)).astype(np.int64)[order.astype(np.int64)]
+        cuts = [0] + [i for i in range(1, m) if v0[i] != v0[i - 1] or i % 7 == 0] + [m]
+        fl = np.zeros(m, dtype=np.uint8)
+        fl[::5] = 1
+        out = R.solve(rg, waves_per_cu=wpc, pack=True, pack_waves_per_cu=2, root_order=order, eager=fl,
+                      runs=np.array(cuts, dtype=np.uint64), want_root_stats=True)
+        return same(out, ref)
+    yield "pack routed runs", pack_routed_runs
+
+    def pack_batch():
+        b = R.Batch(bgs, waves_per_cu=wpc, pack=True, pack_waves_per_cu=2)
+        out = b.run()
+        b.free()
+        for i, bg in enumerate(bgs):
+            ref = oracle_for(bg, root_nodes=False)
+            if (int(out.count[i]), int(out.hash[i]), out.hist_dict(i)) != (ref.count, ref.hash, ref.hist):
+                return f"packed batch graph {i} differs"
+        return None
+    yield "pack batch", pack_batch
+
 
 def main():
     flt = sys.argv[1] if len(sys.argv) > 1 else ""

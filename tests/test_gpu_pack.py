@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import rdmce_amd as R
+from rdmce_amd._lib import MceError
 from graphs import closed_forms, expected_er_cliques, random_graph
 from oracle_lib import OracleGraph, clique_hash
 
@@ -141,3 +142,47 @@ def test_full_size_golden_packed(name, golden):
     assert out.hist_dict() == {int(k): v for k, v in gd["hist"].items()}
     assert out.total_nodes == gd["tree_nodes"]
     banner(f"pack {name}: kernel {out.kernel_ms:.1f} ms")
+
+
+def test_autotune_packed_keeps_results():
+    banner("pack autotune")
+    g = R.Graph.from_edges(random_graph(300, 0.45, 32))
+    g.SortByOrder(R.OrderType.DEG)
+    ref = oracle_for(g)
+    plan = R.Plan(g, pack=True, want_root_stats=True)
+    t = plan.autotune(stride=2)
+    assert t["trials"] >= 8 and t["best_ms"] <= t["base_ms"] * 1.0001
+    assert t["pack_slots"] in (1, 2, 4, 8) and t["pack_waves_per_cu"] in (12, 16, 20, 24)
+    check_same(plan.run(), ref)
+    plan.free()
+
+
+def test_learned_group_schedule_packed():
+    """Learned run order, run splits and routed roots on a graph whose roots
+    vary widely in cost: same results and per-root node counts."""
+    from rdmce_amd import learned
+    banner("pack learned group schedule")
+    g = R.Graph.rmat(14, 16, 0.57, 0.19, 0.19, seed=7)
+    g.SortByOrder(R.OrderType.DEG)
+    m = g.GetNumEdges()
+    ref = oracle_for(g)
+    pred, model, rep = learned.fit_and_predict(g, m, stride=4)
+    model.free()
+    v0 = (g.roots(R.ROOT_EDGE) >> np.uint64(32)).astype(np.int64)
+    sc = learned.learned_group_schedule(pred, np.arange(m, dtype=np.uint64), v0, 64, chunk_share=0.05,
+                                        route_share=0.2)
+    assert sc.eager.sum() > 0 and sc.runs.size - 1 > np.unique(v0).size  # some routed, some runs split
+    out = R.solve(g, root_order=sc.order, eager=sc.eager, runs=sc.runs, pack=True, want_root_stats=True)
+    check_same(out, ref)
+
+
+def test_runs_must_share_a_first_vertex():
+    banner("pack runs validation")
+    g = R.Graph.from_edges(random_graph(60, 0.5, 2))
+    g.SortByOrder(R.OrderType.DEG)
+    m = g.GetNumEdges()
+    order = np.arange(m, dtype=np.uint64)[::-1].copy()
+    with pytest.raises(MceError, match="mixes first vertices"):
+        R.Plan(g, root_order=order, runs=np.array([0, m], dtype=np.uint64), pack=True)
+    with pytest.raises(MceError, match="start at 0"):
+        R.Plan(g, root_order=order, runs=np.array([1, m], dtype=np.uint64), pack=True)

@@ -83,12 +83,13 @@ def test_batch10k_heavy_sample_one_launch():
 
 
 @pytest.mark.parametrize("name,golden,tune_stride", [("cl4m-core300", "cl4m_s4_core300.json", 8),
-                                                      ("rmat24-core500", "rmat24_s5_core500.json", 32)])
+                                                      ("rmat24-core500", "rmat24_s5_core500.json", 16)])
 def test_learned_and_tuned_schedule_full_size(name, golden, tune_stride):
     """The path the benchmark times: per-root costs learned by the GPU cost
-    model from a sampled plain run, the learned order (predicted-heaviest
-    first) with eager splits, and the settings the autotuner picks; the
-    results equal the oracle golden of the whole workload."""
+    model from a sampled plain run, the learned run order, splits and routed
+    roots (learned_group_schedule), the packed search with the settings the
+    autotuner picks; the results equal the oracle golden of the whole
+    workload."""
     gd = load_golden(golden)
     w = workloads.load(name)
     assert w.n_roots == gd["roots"]
@@ -96,14 +97,15 @@ def test_learned_and_tuned_schedule_full_size(name, golden, tune_stride):
     pred, model, rep = learned.fit_and_predict(w.graph, w.n_roots)
     model.free()
     assert np.all(np.isfinite(pred))
-    sched = learned.learned_schedule(pred, np.arange(w.n_roots, dtype=np.uint64), heavy_fraction=0.01,
-                                     eager_top=64)
+    v0 = (w.graph.roots(R.ROOT_EDGE) >> np.uint64(32)).astype(np.int64)
+    sched = learned.learned_group_schedule(pred, np.arange(w.n_roots, dtype=np.uint64), v0, 4096)
     assert sched.learned and sched.order.size == w.n_roots
-    plan = R.Plan(w.graph, root_order=sched.order, eager=sched.eager)
+    plan = R.Plan(w.graph, root_order=sched.order, eager=sched.eager, runs=sched.runs, pack=True)
     try:
         banner(f"{name}: autotune (stride {tune_stride})")
-        plan.autotune(stride=tune_stride)
-        banner(f"{name}: tuned run")
+        tune = plan.autotune(stride=tune_stride)
+        assert tune["pack_slots"] in (1, 2, 4, 8) and tune["pack_waves_per_cu"] in (12, 16, 20, 24)
+        banner(f"{name}: tuned run {tune}")
         out = plan.run()
     finally:
         plan.free()
@@ -111,3 +113,25 @@ def test_learned_and_tuned_schedule_full_size(name, golden, tune_stride):
     assert out.hash == gd["hash"]
     assert out.hist_dict() == {int(k): v for k, v in gd["hist"].items()}
     assert out.total_nodes == gd["tree_nodes"]
+
+
+def test_batch10k_heavy_sample_packed():
+    """The heavy-graph sample of the batch family through the packed search
+    in one launch: per-graph count, hash and histogram equal the oracle's."""
+    gd = load_golden("batch10k_s3_heavy_sample.json")
+    graphs = []
+    for rec in gd["graphs"]:
+        n, p, s = R.batch_graph_params(3, rec["index"], 32, 1024, 0.05, 0.5)
+        g = R.Graph.erdos_renyi(n, p, s)
+        g.SortByOrder(R.OrderType.DEG)
+        graphs.append(g)
+    banner(f"batch10k heavy sample, packed: {len(graphs)} graphs")
+    b = R.Batch(graphs, pack=True)
+    try:
+        out = b.run()
+    finally:
+        b.free()
+    for i, rec in enumerate(gd["graphs"]):
+        assert int(out.count[i]) == rec["count"], rec["index"]
+        assert int(out.hash[i]) == rec["hash"], rec["index"]
+        assert out.hist_dict(i) == {int(k): v for k, v in rec["hist"].items()}, rec["index"]
