@@ -284,6 +284,33 @@ mce_status mce_cost_model_train(mce_cost_model *m, int32_t device, const float *
 mce_status mce_cost_model_predict(const mce_cost_model *m, int32_t device, const float *X, uint64_t n, float *out);
 void mce_cost_model_free(mce_cost_model *m);
 
+/* ------------------------------------------------------------------------ */
+/* Enumeration that allocates nothing (no counterpart in the reference).      */
+/* For callers that cannot allocate device memory: code under HIP graph       */
+/* capture, tight serving loops, harnesses that forbid hipMalloc.  Every      */
+/* scratch buffer lives in one static device array of                          */
+/* MCE_NOALLOC_SCRATCH_BYTES per device, held until the process ends; calls    */
+/* that overlap on one device race on it.                                      */
+/* ------------------------------------------------------------------------ */
+#define MCE_NOALLOC_SCRATCH_BYTES (64ull << 20)
+#define MCE_NOALLOC_ERR_SCRATCH (1ull << 31)
+/* mce_solve_csr (the one-root-per-wavefront kernel, reference dispatch order,
+ * default settings) on a symmetric CSR already in device memory: rowoff
+ * u64[n + 1], col u32[nnz] (rows ascending, no self loops; degeneracy order
+ * keeps per-root scratch small), label_hash u64[n] (splitmix64 of each
+ * vertex's original id, for the clique hash).  Enqueued on `stream` (a
+ * hipStream_t; NULL = the null stream) with no host synchronisation; when
+ * it has run, device memory `out` (caller-owned, 4 + hist_len u64) holds
+ * out[0] = maximal cliques, out[1] = clique hash, out[2] = search-tree nodes,
+ * out[3] = 0, or error bits when the result is incomplete (1: a root with more
+ * than 4096 candidates; 2: a clique larger than hist_len - 1, counted in the
+ * last entry; MCE_NOALLOC_ERR_SCRATCH: a root needed more scratch than the
+ * array holds), out[4 + k] = cliques of size k.  Returns MCE_ERR_UNSUPPORTED
+ * when the graph's derived arrays do not fit the array (about n + m < 2.5M). */
+mce_status mce_solve_device_csr_noalloc(int32_t device, uint64_t n, uint64_t nnz, const uint64_t *rowoff,
+                                        const uint32_t *col, const uint64_t *label_hash, uint32_t hist_len,
+                                        void *stream, uint64_t *out);
+
 int32_t mce_device_count(void);
 mce_status mce_device_sync(int32_t device);
 /* Measurement helper (no reference counterpart): times `iters` launches of
