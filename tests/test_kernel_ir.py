@@ -23,14 +23,16 @@ HIPCC = "/opt/rocm/bin/hipcc"
 DYNAMIC_VEC = re.compile(r"(extractelement <[^>]+> [^,]+, i(32|64) %|insertelement <[^>]+> [^,]+, [^,]+, i(32|64) %)")
 
 
-@pytest.mark.parametrize("src", ["mce_kernel.hip", "mlp.hip"])
+@pytest.mark.parametrize("src", ["mce_kernel.hip", "pack_kernel.hip", "noalloc.cpp", "mlp.hip"])
 def test_no_private_arrays_indexed_at_run_time(tmp_path, src):
     if not os.path.exists(HIPCC) or shutil.which("hipcc") is None and not os.path.exists(HIPCC):
         pytest.skip("hipcc not available")
     out = tmp_path / (src + ".ll")
+    lang = ["-x", "hip"] if src.endswith(".cpp") else []
     r = subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-emit-llvm",
                         "-S", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC,
-                        "-o", str(out), os.path.join(CSRC, src)], capture_output=True, text=True, timeout=600)
+                        "-o", str(out)] + lang + [os.path.join(CSRC, src)], capture_output=True, text=True,
+                       timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     ir = out.read_text()
     allocas = [ln for ln in ir.splitlines() if re.search(r"=\s*alloca\b", ln)]
