@@ -426,6 +426,10 @@ struct mce_plan {
   uint64_t *d_pk_stack = nullptr;
   uint32_t *d_pk_plist = nullptr;
   uint64_t *d_wide[2] = {nullptr, nullptr};
+  uint64_t *d_wide_h = nullptr;     // per entry of the first list: its kept vertex bitmap in d_pool
+  uint64_t *d_pool = nullptr;
+  uint64_t pool_words = 0;
+  uint64_t pool_bound = 0;           // words that every vertex's bitmap would take
   uint64_t wide_cap = 0;
   std::vector<uint64_t> h_rfirst;    // edge roots: canonical id of the first root of each vertex, + m
   uint64_t *d_rfirst = nullptr;
@@ -443,7 +447,7 @@ struct mce_batch {
 
 namespace {
 enum Ctr { kCount, kHash, kNodes, kNext, kListCursor, kNDeferred, kDeferNeed, kDeferStackNeed, kShared, kActive,
-           kIdle, kErr, kWords, kBuild, kPublished, kOwnerTasks, kNext2, kNext3, kNWide1, kNWide2, kNumCtr };
+           kIdle, kErr, kWords, kBuild, kPublished, kOwnerTasks, kNext2, kNext3, kNWide1, kNWide2, kPoolUsed, kNumCtr };
 
 template <class T>
 hipError_t dalloc(T **p, uint64_t count) {
@@ -467,7 +471,7 @@ void plan_release(mce_plan *p) {
                   p->d_slots2,   p->d_stacks,      p->d_stacks2,    p->d_recs,        p->d_users,
                   p->d_has_work, p->d_root_graph,  p->d_gcount,     p->d_ghash,       p->d_ghist,
                   p->d_idle,     p->d_pk_stack,    p->d_pk_plist,   p->d_wide[0],     p->d_wide[1],
-                  p->d_rfirst,   p->d_gstart};
+                  p->d_wide_h,   p->d_pool,        p->d_rfirst,     p->d_gstart};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (auto &e : p->ev)
@@ -593,6 +597,12 @@ mce_status plan_create_impl(const PlanSpec &spec, const mce_options &opt, mce_pl
       }
     });
     outoff[g.n] = g.rowoff[g.n];
+    // words of one kept vertex bitmap per vertex with 2 to 1024 edge roots
+    // (the packed search's pool bound)
+    for (uint64_t v = 0; v < g.n; ++v) {
+      const uint64_t d = g.rowoff[v + 1] - outoff[v], x = outoff[v] - g.rowoff[v];
+      if (d >= 2 && d <= 1024) p->pool_bound += 8 + (d + 63) / 64 * (align_up(d + x, 64) + 1);
+    }
     PTRY(dalloc(&p->d_outoff, g.n + 1));
     PTRY(hipMemcpy(p->d_outoff, outoff.data(), (g.n + 1) * 8, hipMemcpyHostToDevice));
   }
@@ -798,6 +808,13 @@ mce_status ensure_pack(mce_plan *p) {
     p->wide_cap = std::max<uint64_t>(p->n_dispatch, 1);
     HIP_TRY(dalloc(&p->d_wide[0], p->wide_cap));
     HIP_TRY(dalloc(&p->d_wide[1], p->wide_cap));
+    HIP_TRY(dalloc(&p->d_wide_h, p->wide_cap));
+    // Kept vertex bitmaps: at most one per vertex with two or more edge
+    // roots, within a quarter of the free device memory.
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    p->pool_words = std::min<uint64_t>(p->pool_bound, free_b / 4 / 8);
+    if (p->pool_words) HIP_TRY(dalloc(&p->d_pool, p->pool_words));
   }
   if (p->opt.want_list && !p->d_pk_plist) HIP_TRY(dalloc(&p->d_pk_plist, (uint64_t)waves * kPackMaxSlots * kPackMaxP));
   return MCE_OK;
@@ -810,7 +827,8 @@ uint32_t pack_waves(const mce_plan *p) {
 // Resets every counter, result and sharing word of a run with fill kernels
 // (graph-capturable) and launches the enumeration kernels: with the packed
 // search on, pack<1,1> over the dispatch list, pack<2,2> over the roots it
-// found too wide, and the one-root-per-wave kernel over the rest (each
+// found too wide (derived from the vertex bitmaps pack<1,1> kept for them),
+// and the one-root-per-wave kernel over the rest (each
 // later launch reads its list length from the counter the earlier one
 // filled, so the sequence needs no host round trip).
 hipError_t enqueue_run(const mce_plan *p, const KArgs &a, hipStream_t s) {
@@ -847,9 +865,15 @@ hipError_t enqueue_run(const mce_plan *p, const KArgs &a, hipStream_t s) {
     b.n_wide = p->d_ctr + (v == 0 ? kNWide1 : kNWide2);
     b.heavy = p->d_wide[1];
     b.n_heavy = p->d_ctr + kNWide2;
-    if (v == 1) b.eager = nullptr;
-    if (v == 1) {
+    b.pk_pool = p->d_pool;
+    b.pk_pool_words = p->pool_words;
+    b.pk_pool_used = p->d_ctr + kPoolUsed;
+    if (v == 0) {  // wide roots leave with a copy of their vertex bitmap ...
+      b.wide_h = p->d_wide_h;
+    } else {       // ... from which the next launch derives them
+      b.eager = nullptr;
       b.pk_gstart = nullptr;
+      b.pk_whandle = p->d_wide_h;
       b.perm = p->d_wide[0];
       b.reverse = 0;
       b.n_dispatch_dev = p->d_ctr + kNWide1;

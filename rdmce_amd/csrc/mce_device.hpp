@@ -128,6 +128,16 @@ struct KArgs {
   uint32_t *pk_plist;               // per wave pk_slots * kPackMaxP P0 vertex ids (clique lists)
   uint64_t *wide;
   unsigned long long *n_wide;
+  // Vertex bitmaps kept for the next launch (nullable): a run of edge roots
+  // that met wide roots copies its vertex bitmap to pk_pool (pk_pool_words
+  // words; pk_pool_used taken, may overshoot) and names the copy's word
+  // offset (~0: none) in wide_h beside each of those roots; the next launch
+  // reads the same list as pk_whandle, one entry per dispatch entry.
+  uint64_t *wide_h;
+  const uint64_t *pk_whandle;
+  uint64_t *pk_pool;
+  uint64_t pk_pool_words;
+  unsigned long long *pk_pool_used;
   // Edge roots in runs sharing their first vertex: run r covers dispatch
   // entries [pk_gstart[r], pk_gstart[r + 1]) (nullable: single entries);
   // rfirst[v] = canonical id of the first edge root of vertex v.
@@ -144,9 +154,8 @@ struct KArgs {
 // Packed search variants: |P0| <= 64 NW and |X0'| <= 64 NX.
 constexpr uint32_t kPackMaxP = 128;
 constexpr uint32_t kPackMaxSlots = 8;
-// Record words of variant (nw, nx): R P XP V (nw each), XX (nx), key, meta,
-// remaining candidates (nw), rounded up to an even count.
-constexpr uint32_t pack_rec_words(uint32_t nw, uint32_t nx) { return (5 * nw + nx + 2 + 1) / 2 * 2; }
+// Record words of variant (nw, nx): P XP V R (nw each), XX (nx), key, meta.
+constexpr uint32_t pack_rec_words(uint32_t nw, uint32_t nx) { return 4 * nw + nx + 2; }
 // LDS words of one resident subproblem: rows of P0 and X0' over P0 (nw words
 // each), columns of the X0' block per P0 vertex (nx words), label hashes.
 constexpr uint32_t pack_slot_words(uint32_t nw, uint32_t nx) { return 64 * (nw * nw + 2 * nw * nx + nw); }

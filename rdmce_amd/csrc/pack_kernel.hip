@@ -33,7 +33,10 @@
 // consumed:  rows of P0 and of X0' over P0 (NW words), the columns of the X0'
 // block for each P0 vertex (NX words: XX' = XX & cols(t) is one load), and
 // the label hashes of P0.  Roots with |P0| > 64 NW or |X0'| > 64 NX are
-// appended to a.wide for the next launch of the run.
+// appended to a.wide for the next launch of the run.  When a.wide_h is set,
+// the vertex bitmap of a run that met such roots is copied to a.pk_pool
+// (while it has room) and the next launch derives each of them from that
+// copy (a.pk_whandle) instead of building it again.
 #include "mce_common.hpp"
 
 extern __shared__ uint64_t lds_dyn[];
@@ -48,9 +51,12 @@ struct PackCfg {
   static constexpr uint32_t RS = pack_rec_words(NW, NX);
   static constexpr uint32_t SW = pack_slot_words(NW, NX);
   static constexpr uint32_t HS = pack_hist_words(NW);
-  // record word offsets
-  static constexpr uint32_t oR = 0, oP = NW, oX = 2 * NW, oV = 3 * NW, oXX = 4 * NW, oKey = 4 * NW + NX,
-                            oMeta = 4 * NW + NX + 1, oRem = 4 * NW + NX + 2;
+  // record fields (each a column of pk_cap words in the wave's stack):
+  // P XP V (NW each), XX (NX), key, meta, R (NW, read and written only for
+  // clique lists).  meta = first << 32 | root flag | slot << 16 | |R|, where
+  // the record's remaining candidates are the members of V from `first` on.
+  static constexpr uint32_t oP = 0, oX = NW, oV = 2 * NW, oXX = 3 * NW, oKey = 3 * NW + NX, oMeta = 3 * NW + NX + 1,
+                            oR = 3 * NW + NX + 2;
   // slot word offsets
   static constexpr uint32_t sProw = 0, sXrow = 64 * NW * NW, sXcol = sXrow + 64 * NX * NW, sG = sXcol + 64 * NW * NX;
 };
@@ -214,8 +220,8 @@ __device__ __forceinline__ void pk_emit_base(const PackWave<NW, NX> &w, uint64_t
 
 // Builds root `canon` into slot s (the whole wave).  Returns 1 when the root
 // occupies the slot, 0 when it needs no search (its 2-clique, if maximal,
-// is counted here), 2 when it is too wide for this variant (appended to
-// a.wide).
+// is counted here), 2 when it is too wide for this variant (the caller hands
+// it to the next launch).
 template <int NW, int NX>
 __device__ __forceinline__ uint32_t pk_build(const PackWave<NW, NX> &w, uint8_t *scratch, uint32_t s, uint64_t canon,
                                              Acc &acc, uint64_t &cnt, uint64_t &hsh) {
@@ -234,7 +240,6 @@ __device__ __forceinline__ uint32_t pk_build(const PackWave<NW, NX> &w, uint8_t 
     return 0;
   }
   if (p > 64u * NW || p + x > ccap) {
-    if (l == 0) a.wide[atomicAdd(a.n_wide, 1ull)] = canon;
     return 2;
   }
   wave_fence();
@@ -243,7 +248,6 @@ __device__ __forceinline__ uint32_t pk_build(const PackWave<NW, NX> &w, uint8_t 
   const uint64_t xq = compact_x(a, P, p, X, x, above);
   wave_fence();
   if (xq > 64u * NX) {
-    if (l == 0) a.wide[atomicAdd(a.n_wide, 1ull)] = canon;
     return 2;
   }
   const uint64_t rows = p + xq;
@@ -285,6 +289,7 @@ __device__ __forceinline__ uint32_t pk_build(const PackWave<NW, NX> &w, uint8_t 
 // exactly the one pk_build makes, so the search tree is unchanged.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kGroupMaxW = 16;  // |U| <= 1024
+constexpr uint32_t kPoolHdr = 8;     // kept vertex bitmap: v0 d0 W0 xq0 rp o0 rfirst, then T and cm
 
 struct PkGroup {
   uint32_t v0, d0, W0, xq0, rp;
@@ -292,12 +297,14 @@ struct PkGroup {
   uint64_t rfirst;      // canonical id of the edge root (v0, U[0])
   const uint64_t *T;    // W0 x rp words: rows 0..d0-1 (U), d0..d0+xq0-1 (X0v')
   const uint64_t *cm;   // W0 words: OR of the X0v' rows (U vertices with a neighbour below v0)
+  uint64_t *wide;       // the run's roots too wide for this variant, kept until the run ends
 };
 
-// Builds the bitmap of vertex root v0 into `scratch` (the whole wave).
-// Returns false when it does not fit (the caller then builds each edge root
-// on its own).
-__device__ __forceinline__ bool pk_group_build(const KArgs &a, uint8_t *scratch, uint32_t v0, PkGroup &G, Acc &acc) {
+// Builds the bitmap of vertex root v0 into `scratch` (the whole wave), with
+// room for `run` wide root ids after it.  Returns false when it does not fit
+// (the caller then builds each edge root on its own).
+__device__ __forceinline__ bool pk_group_build(const KArgs &a, uint8_t *scratch, uint32_t v0, uint64_t run, PkGroup &G,
+                                               Acc &acc) {
   const uint32_t l = lane();
   const uint64_t b0 = a.rowoff[v0], e0 = a.rowoff[(uint64_t)v0 + 1], o0 = a.outoff[v0];
   const uint64_t d0 = e0 - o0, x0 = o0 - b0;
@@ -306,7 +313,8 @@ __device__ __forceinline__ bool pk_group_build(const KArgs &a, uint8_t *scratch,
   const uint64_t t_off = align_up(4 * x0, 256);
   const uint64_t rp_max = align_up(d0 + x0, 64);
   const uint64_t cm_off = t_off + align_up(8 * W0 * rp_max, 256);
-  if (cm_off + 8 * W0 > a.slot_bytes) return false;
+  const uint64_t wide_off = cm_off + align_up(8 * W0, 256);
+  if (wide_off + 8 * run > a.slot_bytes) return false;
   uint32_t *Xs = (uint32_t *)scratch;
   for (uint64_t i = l; i < x0; i += 64) Xs[i] = a.col[b0 + i];
   wave_fence();
@@ -342,6 +350,7 @@ __device__ __forceinline__ bool pk_group_build(const KArgs &a, uint8_t *scratch,
   G.rfirst = a.rfirst[v0];
   G.T = T;
   G.cm = cm;
+  G.wide = (uint64_t *)(scratch + wide_off);
   return true;
 }
 
@@ -398,7 +407,6 @@ __device__ __forceinline__ uint32_t pk_derive(const PackWave<NW, NX> &w, const P
     return 0;
   }
   if (p > 64u * NW) {
-    if (l == 0) a.wide[atomicAdd(a.n_wide, 1ull)] = canon;
     return 2;
   }
   // X0' rows, in vertex order: rows x of X0v' adjacent to u_k, then U rows
@@ -442,7 +450,6 @@ __device__ __forceinline__ uint32_t pk_derive(const PackWave<NW, NX> &w, const P
   }
   words_read += (uint64_t)nxu * G.W0;
   if (xq > 64u * NX) {
-    if (l == 0) a.wide[atomicAdd(a.n_wide, 1ull)] = canon;
     return 2;
   }
   wave_fence();
@@ -538,7 +545,13 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
 
   const uint64_t n_dispatch = a.n_dispatch_dev ? ld_relaxed((unsigned long long *)a.n_dispatch_dev) : a.n_dispatch;
   uint8_t *scratch = a.slots + (uint64_t)w.me * a.slot_bytes;
+  // The wave's record stack, field-major: field f of record r is
+  // stk[f * cap + r], so lanes touching neighbouring records touch
+  // neighbouring words.
   uint64_t *stk = a.pk_stack + (uint64_t)w.me * a.pk_cap * C::RS;
+  const uint64_t cap = a.pk_cap;
+  const bool lists = a.list != nullptr;
+  auto fld = [stk, cap](uint32_t f, uint32_t r) -> uint64_t & { return stk[(uint64_t)f * cap + r]; };
   ldsu32 *map32 = (ldsu32 *)w.L.map;
   Acc acc{0, 0, 0, 0, 0};
   uint64_t cnt = 0, hsh = 0, words = 0, nodes = 0;  // per lane
@@ -552,11 +565,42 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
   // the vertex bitmap of its v0 (pk_group_build).
   uint64_t gcur = 0, gend = 0;
   bool grouped = false;
+  uint32_t nwide = 0;  // wide roots of the current grouped run, in G.wide
   PkGroup G{};
   for (;;) {
     // Refill: build roots into free slots while fewer than 64 tasks wait.
     while (roots_left && pending < 64 && busy != full) {
       if (gcur == gend) {
+        if (nwide) {  // the finished run's wide roots go out with a copy of its vertex bitmap
+          wave_fence();
+          const uint64_t tw = (uint64_t)G.W0 * G.rp, need = kPoolHdr + tw + G.W0;
+          unsigned long long at = 0, hp = 0;
+          if (l == 0) {
+            at = atomicAdd(a.n_wide, (unsigned long long)nwide);
+            hp = atomicAdd(a.pk_pool_used, (unsigned long long)need);
+          }
+          at = bcast64(at);
+          hp = bcast64(hp);
+          const bool kept = hp + need <= a.pk_pool_words;
+          if (kept) {
+            uint64_t *dst = a.pk_pool + hp;
+            if (l < kPoolHdr) {
+              const uint64_t hv[kPoolHdr] = {G.v0, G.d0, G.W0, G.xq0, G.rp, G.o0, G.rfirst, 0};
+              uint64_t v = 0;
+#pragma unroll
+              for (uint32_t f = 0; f < kPoolHdr; ++f) v = f == l ? hv[f] : v;
+              dst[l] = v;
+            }
+            for (uint64_t i = l; i < tw; i += 64) dst[kPoolHdr + i] = G.T[i];
+            for (uint64_t i = l; i < G.W0; i += 64) dst[kPoolHdr + tw + i] = G.cm[i];
+            acc.build += 8 * need;
+          }
+          for (uint32_t i = l; i < nwide; i += 64) {
+            a.wide[at + i] = G.wide[i];
+            a.wide_h[at + i] = kept ? hp : ~0ull;
+          }
+          nwide = 0;
+        }
         unsigned long long gi = 0;
         if (l == 0) gi = atomicAdd(a.next_root, 1ull);
         gi = bcast64(gi);
@@ -569,7 +613,7 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
         grouped = false;
         if (gend - gcur >= 2 && a.root_mode == 2) {
           const uint64_t c0 = a.perm ? a.perm[gcur] : (a.reverse ? n_dispatch - 1 - gcur : gcur);
-          grouped = pk_group_build(a, scratch, (uint32_t)(a.roots[c0] >> 32), G, acc);
+          grouped = pk_group_build(a, scratch, (uint32_t)(a.roots[c0] >> 32), a.wide_h ? gend - gcur : 0, G, acc);
         }
       }
       const uint64_t i = gcur++;
@@ -579,14 +623,38 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
         continue;
       }
       const uint32_t s = (uint32_t)__builtin_ctzll((uint64_t)(~busy & full));
-      const uint32_t built = grouped ? pk_derive<NW, NX>(w, G, s, canon, acc, cnt, hsh)
-                                     : pk_build<NW, NX>(w, scratch, s, canon, acc, cnt, hsh);
+      bool derive = grouped;
+      const uint64_t hk = a.pk_whandle ? a.pk_whandle[i] : ~0ull;
+      if (hk != ~0ull) {  // the vertex bitmap an earlier launch kept for this root
+        const uint64_t *hdr = a.pk_pool + hk;
+        G.v0 = (uint32_t)hdr[0];
+        G.d0 = (uint32_t)hdr[1];
+        G.W0 = (uint32_t)hdr[2];
+        G.xq0 = (uint32_t)hdr[3];
+        G.rp = (uint32_t)hdr[4];
+        G.o0 = hdr[5];
+        G.rfirst = hdr[6];
+        G.T = hdr + kPoolHdr;
+        G.cm = G.T + (uint64_t)G.W0 * G.rp;
+        derive = true;
+      }
+      const uint32_t built = derive ? pk_derive<NW, NX>(w, G, s, canon, acc, cnt, hsh)
+                                    : pk_build<NW, NX>(w, scratch, s, canon, acc, cnt, hsh);
+      if (built == 2) {  // too wide for this variant: to the next launch
+        if (grouped && a.wide_h) {
+          if (l == 0) G.wide[nwide] = canon;
+          ++nwide;
+        } else if (l == 0) {
+          const unsigned long long at = atomicAdd(a.n_wide, 1ull);
+          a.wide[at] = canon;
+          if (a.wide_h) a.wide_h[at] = ~0ull;
+        }
+      }
       if (built == 1) {
-        if (l == 0) {
-          uint64_t *rec = stk + (uint64_t)top * C::RS;
-          for (uint32_t q = 0; q < C::RS; ++q) rec[q] = 0;
-          rec[C::oMeta] = ((uint64_t)s << 16) | kRecRoot;
-          rec[C::oRem] = 1;
+        if (l == 0) {  // one task (V = {0}): the root's top node
+          for (uint32_t f = 0; f < C::RS; ++f) fld(f, top) = 0;
+          fld(C::oV, top) = 1;
+          fld(C::oMeta, top) = ((uint64_t)s << 16) | kRecRoot;
         }
         wave_fence();
         ++top;
@@ -600,19 +668,22 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
     // record top-1-i; task lane l takes candidate k of look-record ri.
     const uint32_t nlook = top < 64 ? top : 64;
     uint32_t ci = 0;
-    uint64_t lrem[NW];
-    uint32_t lslot = 0;
+    uint64_t lrem[NW], lmeta = 0;
 #pragma unroll
     for (int q = 0; q < NW; ++q) lrem[q] = 0;
     if (l < nlook) {
-      const uint64_t *rec = stk + (uint64_t)(top - 1 - l) * C::RS;
+      const uint32_t r = top - 1 - l;
+      lmeta = fld(C::oMeta, r);
+      const uint32_t first = (uint32_t)(lmeta >> 32);
 #pragma unroll
       for (int q = 0; q < NW; ++q) {
-        lrem[q] = rec[C::oRem + q];
+        const uint32_t lo = 64u * q;
+        const uint64_t keep = first <= lo ? ~0ull : (first >= lo + 64 ? 0ull : ~0ull << (first - lo));
+        lrem[q] = fld(C::oV + q, r) & keep;
         ci += (uint32_t)__popcll(lrem[q]);
       }
-      lslot = (uint32_t)(rec[C::oMeta] >> 16) & 0xFF;
     }
+    const uint32_t lslot = (uint32_t)(lmeta >> 16) & 0xFF;
     const uint32_t incl = wave_scan_add_u32(ci), excl = incl - ci;
     uint32_t T = __builtin_amdgcn_readlane(incl, 63);  // tasks in the top 64 records: min(pending, 64) or more
     T = T < 64 ? T : 64u;
@@ -639,20 +710,23 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
 #pragma unroll
     for (int j = 0; j < NX; ++j) XX[j] = 0;
     if (act) {
-      const uint64_t *rec = stk + (uint64_t)(top - 1 - ri) * C::RS;
+      const uint32_t r = top - 1 - ri;
+      const uint64_t m64 = fld(C::oMeta, r);
+      const uint32_t first = (uint32_t)(m64 >> 32);
       uint64_t rem[NW];
 #pragma unroll
       for (int q = 0; q < NW; ++q) {
-        R[q] = rec[C::oR + q];
-        P[q] = rec[C::oP + q];
-        XP[q] = rec[C::oX + q];
-        V[q] = rec[C::oV + q];
-        rem[q] = rec[C::oRem + q];
+        if (lists) R[q] = fld(C::oR + q, r);
+        P[q] = fld(C::oP + q, r);
+        XP[q] = fld(C::oX + q, r);
+        V[q] = fld(C::oV + q, r);
+        const uint32_t lo = 64u * q;
+        rem[q] = V[q] & (first <= lo ? ~0ull : (first >= lo + 64 ? 0ull : ~0ull << (first - lo)));
       }
 #pragma unroll
-      for (int j = 0; j < NX; ++j) XX[j] = rec[C::oXX + j];
-      key = rec[C::oKey];
-      meta = (uint32_t)rec[C::oMeta];
+      for (int j = 0; j < NX; ++j) XX[j] = fld(C::oXX + j, r);
+      key = fld(C::oKey, r);
+      meta = (uint32_t)m64;
       // t = the k-th remaining candidate
       uint32_t kk = k, base = 0;
 #pragma unroll
@@ -671,7 +745,6 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
     }
     wave_fence();  // every parent is read before records are rewritten below
     if (lpart) {   // the partly taken record keeps its candidates from the (T - excl)-th on
-      uint64_t *rec = stk + (uint64_t)(top - 1 - l) * C::RS;
       uint32_t kk = T - excl, base = 0;
 #pragma unroll
       for (int q = 0; q < NW - 1; ++q) {
@@ -686,12 +759,7 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
       for (int q = 1; q < NW; ++q)
         if (base == (uint32_t)q * 64) rw = lrem[q];
       const uint32_t first = base + select64(rw, kk);
-#pragma unroll
-      for (int q = 0; q < NW; ++q) {
-        const uint32_t lo = 64u * q;
-        const uint64_t keep = first <= lo ? ~0ull : (first >= lo + 64 ? 0ull : ~0ull << (first - lo));
-        rec[C::oRem + q] = lrem[q] & keep;
-      }
+      fld(C::oMeta, top - 1 - l) = (lmeta & 0xFFFFFFFFull) | ((uint64_t)first << 32);
     }
     if (lfull) __hip_atomic_fetch_sub(w.m(mRefs, lslot), (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     top -= nfull;
@@ -886,20 +954,19 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
     const uint32_t npush = (uint32_t)__popcll(bm);
     uint32_t vc = 0;
     if (branch) {
-      uint64_t *rec = stk + (uint64_t)(top + npush - 1 - rank_below(bm)) * C::RS;
+      const uint32_t r = top + npush - 1 - rank_below(bm);
 #pragma unroll
       for (int q = 0; q < NW; ++q) {
-        rec[C::oR + q] = R[q];
-        rec[C::oP + q] = P[q];
-        rec[C::oX + q] = XP[q];
-        rec[C::oV + q] = V[q];
-        rec[C::oRem + q] = V[q];
+        if (lists) fld(C::oR + q, r) = R[q];
+        fld(C::oP + q, r) = P[q];
+        fld(C::oX + q, r) = XP[q];
+        fld(C::oV + q, r) = V[q];
         vc += (uint32_t)__popcll(V[q]);
       }
 #pragma unroll
-      for (int j = 0; j < NX; ++j) rec[C::oXX + j] = XX[j];
-      rec[C::oKey] = key;
-      rec[C::oMeta] = meta;
+      for (int j = 0; j < NX; ++j) fld(C::oXX + j, r) = XX[j];
+      fld(C::oKey, r) = key;
+      fld(C::oMeta, r) = meta;  // first = 0: every candidate remains
       __hip_atomic_fetch_add(w.m(mRefs, s), (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     const uint32_t vsum = __builtin_amdgcn_readlane(wave_scan_add_u32(vc), 63);

@@ -20,7 +20,7 @@ sys.path.insert(0, TESTS)
 import numpy as np  # noqa: E402
 
 import rdmce_amd as R  # noqa: E402
-from graphs import closed_forms, random_graph  # noqa: E402
+from graphs import closed_forms, dense_core_graph, random_graph  # noqa: E402
 from oracle_lib import OracleGraph, clique_hash  # noqa: E402
 
 
@@ -159,6 +159,24 @@ def cases():
                                want_root_stats=True), ref))
     yield "pack vertex two-word", lambda: same(R.solve(tg, waves_per_cu=wpc, root_mode=R.ROOT_VERTEX, pack=True,
                                                        pack_waves_per_cu=2, want_root_stats=True), tref)
+
+    def pack_wide_runs():
+        # one dense vertex's run mixes one- and two-word roots: the two-word
+        # ones reach the second launch as runs derived from one vertex bitmap
+        rg = graph(dense_core_graph(90, 5, 200, 0.05, 11))
+        return same(R.solve(rg, waves_per_cu=wpc, pack=True, pack_waves_per_cu=2, want_root_stats=True),
+                    oracle_for(rg))
+    yield "pack wide runs", pack_wide_runs
+
+    def pack_wide_runs_small_pool():
+        # room for a few kept vertex bitmaps only: the other wide roots are
+        # built on their own by the second launch
+        os.environ["EMU_FREE_BYTES"] = str(4 * 8 * 3000)
+        try:
+            return pack_wide_runs()
+        finally:
+            del os.environ["EMU_FREE_BYTES"]
+    yield "pack wide runs small pool", pack_wide_runs_small_pool
 
     def pack_routed_runs():
         # custom order, explicit runs split mid-vertex, every 5th root routed

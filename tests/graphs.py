@@ -103,6 +103,21 @@ def random_graph(n, p, seed):
     return list(zip(iu[0][sel].tolist(), iu[1][sel].tolist()))
 
 
+def dense_core_graph(k, holes, n_sparse, p, seed):
+    """K_k minus `holes` disjoint edges (2^holes maximal cliques of size
+    k - holes ... ) joined to a sparse G(n_sparse, p) by random edges: the
+    first dense vertex's edge roots mix roots with more than 64 candidates
+    and roots with fewer, in one run sharing that vertex."""
+    rng = np.random.default_rng(seed)
+    cut = {(2 * i, 2 * i + 1) for i in range(holes)}
+    edges = [(i, j) for i in range(k) for j in range(i + 1, k) if (i, j) not in cut]
+    edges += [(k + a, k + b) for a, b in random_graph(n_sparse, p, seed + 1)]
+    for v in range(k, k + n_sparse):
+        for u in rng.choice(k, size=2, replace=False).tolist():
+            edges.append((u, v))
+    return edges
+
+
 def hist_of(cliques):
     h = {}
     for c in cliques:

@@ -13,7 +13,7 @@ import pytest
 
 import rdmce_amd as R
 from rdmce_amd._lib import MceError
-from graphs import closed_forms, expected_er_cliques, random_graph
+from graphs import closed_forms, dense_core_graph, expected_er_cliques, random_graph
 from oracle_lib import OracleGraph, clique_hash
 
 pytestmark = pytest.mark.gpu
@@ -73,6 +73,21 @@ def test_random_graphs_packed(n, p, seed, slots):
     lists = expected_er_cliques(n, p) < 3e5
     out = R.solve(g, pack=True, pack_slots=slots, want_root_stats=True, want_list=lists)
     check_same(out, oracle_for(g, list_cap=(1 << 24) if lists else 0), lists=lists)
+    plain = R.solve(g, want_root_stats=True)
+    assert out.search_bytes == plain.search_bytes
+
+
+@pytest.mark.parametrize("k,slots", [(90, 2), (150, 4)])
+def test_wide_runs_packed(k, slots):
+    """A dense vertex whose run of edge roots mixes one-word, two-word and
+    (k = 150) wider roots: the second launch takes the two-word ones as runs
+    derived from one vertex bitmap, the rest reach the one-root-per-wave
+    kernel."""
+    banner(f"pack wide runs K{k}")
+    g = R.Graph.from_edges(dense_core_graph(k, 5, 400, 0.03, 11))
+    g.SortByOrder(R.OrderType.DEG)
+    out = R.solve(g, pack=True, pack_slots=slots, want_root_stats=True, want_list=True)
+    check_same(out, oracle_for(g, list_cap=1 << 20), lists=True)
     plain = R.solve(g, want_root_stats=True)
     assert out.search_bytes == plain.search_bytes
 
