@@ -814,6 +814,9 @@ mce_status ensure_pack(mce_plan *p) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
     p->pool_words = std::min<uint64_t>(p->pool_bound, free_b / 4 / 8);
+#ifdef RDMCE_PACK_POOL_MAX  // test builds: a pool that runs out after a few bitmaps
+    p->pool_words = std::min<uint64_t>(p->pool_words, RDMCE_PACK_POOL_MAX);
+#endif
     if (p->pool_words) HIP_TRY(dalloc(&p->d_pool, p->pool_words));
   }
   if (p->opt.want_list && !p->d_pk_plist) HIP_TRY(dalloc(&p->d_pk_plist, (uint64_t)waves * kPackMaxSlots * kPackMaxP));

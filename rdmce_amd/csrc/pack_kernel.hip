@@ -379,10 +379,11 @@ __device__ __forceinline__ void pext_row(const PkGroup &G, uint64_t mk, uint32_t
 }
 
 // Derives edge root `canon` = (v0, u_k) of group G into slot s (the whole
-// wave).  Returns as pk_build does.
+// wave).  Returns as pk_build does, or 3 when the root's top node is pruned
+// by an X0' row covering P0 (its search words in `pruned`).
 template <int NW, int NX>
 __device__ __forceinline__ uint32_t pk_derive(const PackWave<NW, NX> &w, const PkGroup &G, uint32_t s, uint64_t canon,
-                                              Acc &acc, uint64_t &cnt, uint64_t &hsh) {
+                                              Acc &acc, uint64_t &cnt, uint64_t &hsh, uint32_t &pruned) {
   using C = PackCfg<NW, NX>;
   const KArgs &a = w.a;
   const uint32_t l = lane();
@@ -450,7 +451,41 @@ __device__ __forceinline__ uint32_t pk_derive(const PackWave<NW, NX> &w, const P
   }
   words_read += (uint64_t)nxu * G.W0;
   if (xq > 64u * NX) {
-    return 2;
+    // Too many X0' rows for this variant.  When one of them covers P0, the
+    // root's top node is pruned (no maximal clique below it): one node, no
+    // slot.  Otherwise the root goes to the next launch.
+    bool cov = false;
+    for (uint32_t base = 0; base < G.xq0; base += 64) {
+      const uint32_t x = base + l;
+      const bool cand = x < G.xq0 && ((G.T[(uint64_t)kq * G.rp + G.d0 + x] >> kb) & 1);
+      uint32_t c = 0;
+      for (uint32_t z = 0; z < G.W0; ++z) {
+        const uint64_t mz = readlane64(pk, z);
+        if (cand) c += (uint32_t)__popcll(G.T[(uint64_t)z * G.rp + G.d0 + x] & mz);
+      }
+      cov |= cand && c == p;
+    }
+    for (uint32_t base = 0; base < nxu; base += 64) {
+      const uint32_t t = base + l;
+      uint32_t j = 0, before = 0;
+      for (uint32_t z = 0; z <= kq; ++z) {
+        const uint64_t wz = readlane64(xu, z);
+        const uint32_t cz = (uint32_t)__popcll(wz);
+        if (t >= before && t < before + cz) j = 64 * z + select64(wz, t - before);
+        before += cz;
+      }
+      uint32_t c = 0;
+      for (uint32_t z = 0; z < G.W0; ++z) {
+        const uint64_t mz = readlane64(pk, z);
+        if (t < nxu) c += (uint32_t)__popcll(G.T[(uint64_t)z * G.rp + j] & mz);
+      }
+      cov |= t < nxu && c == p;
+    }
+    if (__ballot(cov) == 0) return 2;
+    words_read += (uint64_t)(G.xq0 + nxu) * G.W0;
+    acc.build += 8 * words_read;
+    pruned = (p + xq) * ((p + 63) / 64);  // the search words of evaluating the top node
+    return 3;
   }
   wave_fence();
   // Row numbers (in G) of the edge root's rows: P0 (the set bits of pk in
@@ -638,8 +673,14 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
         G.cm = G.T + (uint64_t)G.W0 * G.rp;
         derive = true;
       }
-      const uint32_t built = derive ? pk_derive<NW, NX>(w, G, s, canon, acc, cnt, hsh)
+      uint32_t pruned = 0;
+      const uint32_t built = derive ? pk_derive<NW, NX>(w, G, s, canon, acc, cnt, hsh, pruned)
                                     : pk_build<NW, NX>(w, scratch, s, canon, acc, cnt, hsh);
+      if (built == 3 && l == 0) {  // one tree node, pruned
+        ++nodes;
+        words += pruned;
+        if (a.root_nodes) atomicAdd(&a.root_nodes[canon], 1ull);
+      }
       if (built == 2) {  // too wide for this variant: to the next launch
         if (grouped && a.wide_h) {
           if (l == 0) G.wide[nwide] = canon;

@@ -168,16 +168,6 @@ def cases():
                     oracle_for(rg))
     yield "pack wide runs", pack_wide_runs
 
-    def pack_wide_runs_small_pool():
-        # room for a few kept vertex bitmaps only: the other wide roots are
-        # built on their own by the second launch
-        os.environ["EMU_FREE_BYTES"] = str(4 * 8 * 3000)
-        try:
-            return pack_wide_runs()
-        finally:
-            del os.environ["EMU_FREE_BYTES"]
-    yield "pack wide runs small pool", pack_wide_runs_small_pool
-
     def pack_routed_runs():
         # custom order, explicit runs split mid-vertex, every 5th root routed
         # to the one-root-per-wave kernel

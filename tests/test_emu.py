@@ -42,7 +42,9 @@ def test_emulated_kernels_match_oracle(emu_lib, case):
                                   "pack routed runs"])
 def test_emulated_packed_depth_first_expansion(emu_lib, case):
     """The packed search's bounded-stack mode (only the top record expanded)
-    on every dispatch beyond two stacked records: same results."""
+    on every dispatch beyond two stacked records, and a pool with room for
+    two kept vertex bitmaps (the other wide roots are built on their own by
+    the second launch): same results."""
     lib = os.path.join(EMU, "_build", "librdmce_emu_thr.so")
     env = dict(os.environ, RDMCE_LIB=lib, EMU_CUS="1", EMU_BARRIER_TIMEOUT="120")
     r = subprocess.run([sys.executable, os.path.join(EMU, "run_cases.py"), case], capture_output=True, text=True,
