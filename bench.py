@@ -341,6 +341,14 @@ def main():
     if not same(abl_outs[0], plain):
         raise SystemExit("batched plain-order results differ from the plain path")
 
+    # Root granularity: vertex roots against the edge roots of the same
+    # vertices (every 64th vertex of the covered ones), tuned packed settings.
+    nv = int(v0[w.n_roots - 1]) + 1 if w.n_roots else 0
+    gran = R.choose_root_mode(g, n_vertex_roots=nv, stride=64, pack=True, pack_slots=tune["pack_slots"],
+                              pack_waves_per_cu=tune["pack_waves_per_cu"], share_bound=tune["share_bound"])
+    gran["root_mode"] = "vertex" if gran["root_mode"] == R.ROOT_VERTEX else "edge"
+    log(f"root granularity: {gran}")
+
     if args.copy_probe:
         import ctypes
         from rdmce_amd._lib import check, lib
@@ -372,6 +380,7 @@ def main():
                                       "chunk_share": args.chunk_share, "route_share": args.route_share,
                                       "seconds": round(learn_s, 2)},
                        tuned=dict(tune, seconds=round(tune_s, 2), stride=args.tune_stride),
+                       root_granularity=gran,
                        passes=first.passes, waves=first.waves),
         "plain": plain_info,
         "speedup_vs_plain": value / plain_info["value"],

@@ -7,8 +7,9 @@ the enumeration.
 from ._lib import (MceError, ROOT_EDGE, ROOT_VERTEX, LIB_PATH, lib)  # noqa: F401
 from .graph import Graph, OrderType, batch_graph_params, bin_path_for  # noqa: F401
 from .solver import (Batch, BatchOutput, BkSolverWrapper, MceGpuSolver, MceOutput, Plan,  # noqa: F401
-                     device_count, solve)
+                     choose_root_mode, device_count, solve)
 
 __all__ = ["Graph", "OrderType", "MceGpuSolver", "BkSolverWrapper", "solve", "Plan", "Batch", "BatchOutput",
+           "choose_root_mode",
            "MceOutput", "MceError",
            "ROOT_EDGE", "ROOT_VERTEX", "batch_graph_params", "bin_path_for", "device_count", "lib", "LIB_PATH"]

@@ -153,6 +153,46 @@ class Plan:
         self.free()
 
 
+def choose_root_mode(graph: Graph, n_vertex_roots: int = 0, stride: int = 64, repeats: int = 2, **plan_kw) -> dict:
+    """Root granularity for `graph`: vertex roots (the reference's LEVEL_1)
+    or edge roots (LEVEL_2), BkPivotBitBalance.cuh:414-427.
+
+    Times the vertex roots v of every `stride`-th vertex among the first
+    `n_vertex_roots` (all vertices when 0) against the edge roots (v, u) of
+    the same vertices, with the plan settings `plan_kw` (best of `repeats`
+    runs after one untimed run), and returns the faster mode with both
+    times.  Both root sets enumerate the maximal cliques whose lowest vertex
+    is a sampled v; a differing count or hash raises."""
+    n = graph.GetNumVertices() if n_vertex_roots <= 0 else min(n_vertex_roots, graph.GetNumVertices())
+    vs = np.arange(0, n, max(stride, 1), dtype=np.uint64)
+    v0 = graph.roots(_lib.ROOT_EDGE) >> np.uint64(32)
+    lo = np.searchsorted(v0, vs, side="left")
+    hi = np.searchsorted(v0, vs, side="right")
+    edges = np.concatenate([np.arange(a, b, dtype=np.uint64) for a, b in zip(lo, hi)] or
+                           [np.zeros(0, dtype=np.uint64)])
+    kw = {k: v for k, v in plan_kw.items() if k not in ("root_mode", "root_order", "eager", "runs")}
+    out = {}
+    for label, mode, order in (("vertex", _lib.ROOT_VERTEX, vs), ("edge", _lib.ROOT_EDGE, edges)):
+        if order.size == 0:
+            out[label] = (0.0, 0, 0)
+            continue
+        plan = Plan(graph, root_mode=mode, root_order=order, **kw)
+        try:
+            plan.run()
+            best = None
+            for _ in range(max(repeats, 1)):
+                o = plan.run()
+                best = o.kernel_ms if best is None else min(best, o.kernel_ms)
+            out[label] = (best, o.count, o.hash)
+        finally:
+            plan.free()
+    if out["vertex"][1:] != out["edge"][1:]:
+        raise RuntimeError(f"vertex and edge roots of the sample disagree: {out['vertex'][1:]} != {out['edge'][1:]}")
+    mode = _lib.ROOT_VERTEX if out["vertex"][0] < out["edge"][0] else _lib.ROOT_EDGE
+    return {"root_mode": mode, "vertex_ms": out["vertex"][0], "edge_ms": out["edge"][0],
+            "sample_vertices": int(vs.size), "sample_edge_roots": int(edges.size), "sample_cliques": out["edge"][1]}
+
+
 def _tune_dict(t) -> dict:
     return {"waves_per_cu": t.waves_per_cu, "share_bound": t.share_bound, "lds_tier": t.lds_tier,
             "trials": t.trials, "best_ms": t.best_ms, "base_ms": t.base_ms, "pack_slots": t.pack_slots,

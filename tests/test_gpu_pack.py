@@ -92,6 +92,21 @@ def test_wide_runs_packed(k, slots):
     assert out.search_bytes == plain.search_bytes
 
 
+@pytest.mark.parametrize("pack", [False, True])
+def test_choose_root_mode(pack):
+    """The granularity chooser times vertex roots against the edge roots of
+    the same vertices; both enumerate the same cliques, which it checks."""
+    banner(f"choose_root_mode pack={pack}")
+    g = R.Graph.from_edges(random_graph(400, 0.1, 21))
+    g.SortByOrder(R.OrderType.DEG)
+    ch = R.choose_root_mode(g, stride=3, pack=pack)
+    assert ch["root_mode"] in (R.ROOT_VERTEX, R.ROOT_EDGE)
+    assert ch["vertex_ms"] > 0 and ch["edge_ms"] > 0 and ch["sample_vertices"] == 134
+    og = OracleGraph.from_csr(g.GetRowOffset(), g.GetColIdx(), g.GetLabels(), g.GetDegeneracy())
+    ref = og.mce_roots(np.arange(0, 400, 3, dtype=np.uint64), root_mode=1)
+    assert ch["sample_cliques"] == ref.count
+
+
 def test_er_config1_packed():
     banner("pack ER G(2048, 0.008) seed 1")
     g = R.Graph.erdos_renyi(2048, 0.008, 1)
