@@ -858,10 +858,14 @@ hipError_t enqueue_run(const mce_plan *p, const KArgs &a, hipStream_t s) {
 #ifdef RDMCE_PACK_THR_MAX  // test builds: reach the depth-first expansion on small graphs
     b.pk_thr = std::min<uint32_t>(b.pk_thr, RDMCE_PACK_THR_MAX);
 #endif
-    // the two-word variant's slots are 3.5x larger: half as many, and as
-    // many as one workgroup's LDS holds
+    // the two-word variant's slots are 3.5x larger: half as many; and no more
+    // than let the requested waves per CU all be resident in its LDS (a wave
+    // that waits for LDS starts late and finds little work left)
+    const bool batch = a.root_graph != nullptr;
     uint32_t slots = v == 0 ? p->pack_slots : std::max<uint32_t>(1, p->pack_slots / 2);
-    while (slots > 1 && pack_lds_bytes(nw, nx, slots, a.root_graph != nullptr) > p->lds_per_block_max) --slots;
+    while (slots > 1 && (pack_lds_bytes(nw, nx, slots, batch) > p->lds_per_block_max ||
+                         pack_lds_bytes(nw, nx, slots, batch) * (uint64_t)p->pack_wpc > p->lds_per_cu))
+      --slots;
     b.pk_slots = slots;
     b.pk_plist = p->d_pk_plist;
     b.wide = p->d_wide[v];

@@ -320,18 +320,26 @@ __device__ __forceinline__ uint64_t compact_x(const KArgs &a, const uint32_t *P,
 // adjacency run against P; with fewer than 64 rows, g = 2 or 4 lanes share a
 // row, each matching one share of its P range (a row's cost is a chain of
 // dependent loads into adjacency lists; splitting the range shortens the
-// chain), so callees must set bits with atomic ORs.  Returns this lane's
+// chain), so callees must set bits with atomic ORs.  With `claim` (an LDS
+// word), lanes take the items after their first one from a shared counter,
+// so a lane with short rows goes on to further rows.  Returns this lane's
 // share of the adjacency entries in the runs matched (each row's run counted
 // once, by its first lane).
 template <class PP, class XW>
 __device__ __forceinline__ uint64_t match_rows(const KArgs &a, const uint32_t *P, uint64_t p, const uint32_t *X,
-                                               uint64_t rows, uint32_t above, PP pp, XW xw) {
+                                               uint64_t rows, uint32_t above, PP pp, XW xw, ldsu32 *claim = nullptr) {
   const uint32_t l = lane();
   const uint32_t *col = a.col;
   uint64_t scanned = 0;
   const uint32_t g = rows <= 16 ? 4u : (rows <= 32 ? 2u : 1u);
   const uint32_t gs = g == 4 ? 2u : (g == 2 ? 1u : 0u);
-  for (uint64_t t = l; t < rows << gs; t += 64) {
+  const uint64_t items = rows << gs;
+  if (claim) {  // items past the first 64 go to whichever lane is free first
+    if (l == 0) *claim = 64;
+    wave_fence();
+  }
+  for (uint64_t t = l; t < items;
+       t = claim ? (uint64_t)__hip_atomic_fetch_add(claim, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : t + 64) {
     const uint64_t i = t >> gs;
     const uint32_t sh = (uint32_t)t & (g - 1);
     const uint32_t j0 = i < p ? (uint32_t)i + 1 : 0u;  // the row's P range [j0, p)

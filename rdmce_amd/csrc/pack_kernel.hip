@@ -301,10 +301,11 @@ struct PkGroup {
 };
 
 // Builds the bitmap of vertex root v0 into `scratch` (the whole wave), with
-// room for `run` wide root ids after it.  Returns false when it does not fit
-// (the caller then builds each edge root on its own).
+// room for `run` wide root ids after it; `claim` is an LDS word for handing
+// out rows.  Returns false when it does not fit (the caller then builds each
+// edge root on its own).
 __device__ __forceinline__ bool pk_group_build(const KArgs &a, uint8_t *scratch, uint32_t v0, uint64_t run, PkGroup &G,
-                                               Acc &acc) {
+                                               Acc &acc, ldsu32 *claim) {
   const uint32_t l = lane();
   const uint64_t b0 = a.rowoff[v0], e0 = a.rowoff[(uint64_t)v0 + 1], o0 = a.outoff[v0];
   const uint64_t d0 = e0 - o0, x0 = o0 - b0;
@@ -332,7 +333,7 @@ __device__ __forceinline__ bool pk_group_build(const KArgs &a, uint8_t *scratch,
         or64(&T[(uint64_t)(j >> 6) * rp + i], 1ull << (j & 63));
         or64(&T[(uint64_t)(i >> 6) * rp + j], 1ull << (i & 63));
       },
-      [&](uint64_t i, uint32_t wd, uint64_t bits) { or64(&T[(uint64_t)wd * rp + i], bits); });
+      [&](uint64_t i, uint32_t wd, uint64_t bits) { or64(&T[(uint64_t)wd * rp + i], bits); }, claim);
   wave_fence();
   if (l < W0) {
     uint64_t o = 0;
@@ -648,7 +649,8 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
         grouped = false;
         if (gend - gcur >= 2 && a.root_mode == 2) {
           const uint64_t c0 = a.perm ? a.perm[gcur] : (a.reverse ? n_dispatch - 1 - gcur : gcur);
-          grouped = pk_group_build(a, scratch, (uint32_t)(a.roots[c0] >> 32), a.wide_h ? gend - gcur : 0, G, acc);
+          grouped = pk_group_build(a, scratch, (uint32_t)(a.roots[c0] >> 32), a.wide_h ? gend - gcur : 0, G, acc,
+                                   map32);
         }
       }
       const uint64_t i = gcur++;

@@ -250,6 +250,13 @@ uint32_t emu_readlane(uint32_t v, uint32_t lane) {
 
 uint32_t emu_readfirstlane(uint32_t v) { return (uint32_t)exchange(v)[0]; }
 
+uint32_t emu_bpermute(int addr, uint32_t v) {  // ds_bpermute: the value of lane (addr / 4) mod 64
+  const uint64_t *b = exchange(v);
+  const uint32_t src = ((uint32_t)addr >> 2) & 63;
+  if (src >= t_wave->lanes) emu_fail("bpermute of a lane the wave does not have");
+  return (uint32_t)b[src];
+}
+
 uint32_t emu_update_dpp(uint32_t old, uint32_t src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl) {
   const uint64_t *b = exchange(src);
   const int l = t_lane, row = l >> 4, rl = l & 15;

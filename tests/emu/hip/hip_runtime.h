@@ -132,6 +132,7 @@ uint32_t emu_lane();
 uint64_t emu_ballot(bool pred);
 uint32_t emu_readlane(uint32_t v, uint32_t lane);
 uint32_t emu_readfirstlane(uint32_t v);
+uint32_t emu_bpermute(int addr, uint32_t v);
 uint32_t emu_update_dpp(uint32_t old, uint32_t src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl);
 uint32_t emu_mbcnt_lo(uint32_t mask, uint32_t acc);
 uint32_t emu_mbcnt_hi(uint32_t mask, uint32_t acc);
@@ -152,6 +153,7 @@ inline int emu_ctz(unsigned long long x) {
 #define __builtin_ctzll(x) emu_ctz((x))
 #define __builtin_amdgcn_readlane(v, l) emu_readlane((uint32_t)(v), (uint32_t)(l))
 #define __builtin_amdgcn_readfirstlane(v) emu_readfirstlane((uint32_t)(v))
+#define __builtin_amdgcn_ds_bpermute(a, v) ((int)emu_bpermute((a), (uint32_t)(v)))
 #define __builtin_amdgcn_update_dpp(o, s, c, rm, bm, bc) emu_update_dpp((o), (s), (c), (rm), (bm), (bc))
 #define __builtin_amdgcn_mbcnt_lo(m, a) emu_mbcnt_lo((m), (a))
 #define __builtin_amdgcn_mbcnt_hi(m, a) emu_mbcnt_hi((m), (a))
@@ -173,6 +175,8 @@ inline T *emu_gp(T *p) {
 }
 inline uint64_t *emu_gp(emu_as3_u64 *p) { return (uint64_t *)p; }
 inline unsigned long long *emu_gp(emu_as3_ull *p) { return (unsigned long long *)p; }
+using emu_as3_u32 = __attribute__((address_space(3))) uint32_t;
+inline uint32_t *emu_gp(emu_as3_u32 *p) { return (uint32_t *)p; }
 
 #define __hip_atomic_fetch_add(p, v, o, s) __atomic_fetch_add(emu_gp(p), (v), __ATOMIC_SEQ_CST)
 #define __hip_atomic_fetch_sub(p, v, o, s) __atomic_fetch_sub(emu_gp(p), (v), __ATOMIC_SEQ_CST)

@@ -160,13 +160,14 @@ def cases():
     yield "pack vertex two-word", lambda: same(R.solve(tg, waves_per_cu=wpc, root_mode=R.ROOT_VERTEX, pack=True,
                                                        pack_waves_per_cu=2, want_root_stats=True), tref)
 
-    def pack_wide_runs():
-        # one dense vertex's run mixes one- and two-word roots: the two-word
-        # ones reach the second launch as runs derived from one vertex bitmap
-        rg = graph(dense_core_graph(90, 5, 200, 0.05, 11))
+    def pack_wide_runs(k):
+        # dense vertices whose runs mix one-word, two-word and (k = 150)
+        # wider roots, with vertex bitmaps of up to 2 (k = 90) or 3 words
+        rg = graph(dense_core_graph(k, 5, 200, 0.05, 11))
         return same(R.solve(rg, waves_per_cu=wpc, pack=True, pack_waves_per_cu=2, want_root_stats=True),
                     oracle_for(rg))
-    yield "pack wide runs", pack_wide_runs
+    yield "pack wide runs K90", lambda: pack_wide_runs(90)
+    yield "pack wide runs K150", lambda: pack_wide_runs(150)
 
     def pack_routed_runs():
         # custom order, explicit runs split mid-vertex, every 5th root routed

@@ -16,8 +16,8 @@ LIB = os.path.join(EMU, "_build", "librdmce_emu.so")
 QUICK = ["zachary", "closed K5", "closed moon_moser4", "closed cocktail8", "closed multipartite_1_5_9_2",
          "G(60,0.5) tau=1 lds=1", "G(60,0.5) tau=-1 lds=0", "G(120,0.3) tau=4 lds=0", "G(200,0.08) tau=24 lds=1",
          "vertex roots", "two-word subset", "deferred pass", "batch", "plan graph replay",
-         "pack zachary list", "pack G(60,0.5) slots=1", "pack G(45,0.8) slots=4", "pack wide runs", "pack wide runs small pool", "pack routed runs",
-         "pack batch"]
+         "pack zachary list", "pack G(60,0.5) slots=1", "pack G(45,0.8) slots=4", "pack wide runs K90",
+         "pack wide runs K150", "pack routed runs", "pack batch"]
 
 
 @pytest.fixture(scope="module")
