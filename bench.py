@@ -143,7 +143,26 @@ def roofline(out, kernel_ms: float, workload: str):
             "traffic": read_profile(f"pack_{workload.replace('-', '_')}_pmc", "traffic_bytes_per_step"),
             "t_lb_ms": t_lb * 1e3, "build_bytes": b_build, "search_bytes": b_search, "search_ops": ops,
             "ceilings_ms": {k: v[0] * 1e3 for k, v in ceil.items()},
-            "frac_of_peak": {k: v[1] / v[2] for k, v in ceil.items()}}
+            "frac_of_peak": {k: v[1] / v[2] for k, v in ceil.items()},
+            "per_root_build_formula": per_root_build(workload, t)}
+
+
+def per_root_build(workload: str, t: float):
+    """The per-root build formula 4 (d(v0) + d(v1)) + 4 sum_{u in P0} d(u) +
+    8 rows W, summed over the workload's roots by the oracle
+    (tests/golden/build_terms.json): the bytes a build that scans each
+    candidate's whole adjacency list for every root would move, and the HBM
+    time they would take at peak beside the measured step."""
+    fn = os.path.join(ROOT, "tests", "golden", "build_terms.json")
+    try:
+        with open(fn) as f:
+            terms = json.load(f).get(workload)
+    except OSError:
+        terms = None
+    if not terms:
+        return None
+    hbm_s = terms["build_bytes"] / (HBM_PEAK_GBS * 1e9)
+    return {"build_bytes": terms["build_bytes"], "hbm_ms": hbm_s * 1e3, "over_measured": hbm_s / t}
 
 
 def cpu_model() -> str:

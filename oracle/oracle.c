@@ -545,6 +545,103 @@ static void solve_root(thr_t *t, uint64_t r) {
   if (t->sh->want_nodes && !t->sh->ids) t->sh->root_nodes[r - t->sh->root_begin] = t->root_nodes;
 }
 
+/* Per-root algorithmic build terms of the edge roots [begin, end) (the
+ * survey's measurement formula, with the roots built as solve_root and
+ * solve_sub build them): deg = d(v0) + d(v1), pdeg = sum over u in P0 of
+ * d(u), rows_w = (p + x') * ceil(p / 64), x' the rows of X with a neighbour
+ * in P0.  B_build = 4 deg + 4 pdeg + 8 rows_w.  Test and measurement
+ * infrastructure; not part of the enumeration. */
+typedef struct {
+  const orc_graph *g;
+  const uint64_t *outoff; /* canonical id of the first edge root of each vertex, n + 1 */
+  uint64_t begin, end;
+  _Atomic uint64_t next_v;
+  pthread_mutex_t mu;
+  uint64_t deg, pdeg, rows_w, roots;
+} terms_t;
+
+static void *terms_worker(void *arg) {
+  terms_t *T = arg;
+  const orc_graph *g = T->g;
+  uint64_t deg = 0, pdeg = 0, rows_w = 0, roots = 0, cap = 0;
+  uint32_t *buf = NULL;
+  for (;;) {
+    uint64_t vb = atomic_fetch_add(&T->next_v, 256);
+    if (vb >= g->n) break;
+    uint64_t ve = vb + 256 < g->n ? vb + 256 : g->n;
+    for (uint64_t v0 = vb; v0 < ve; ++v0) {
+      const uint32_t *n0 = g->col + g->rowoff[v0];
+      const uint64_t d0 = g->rowoff[v0 + 1] - g->rowoff[v0];
+      const uint64_t first_out = lower_bound_u32(n0, 0, d0, (uint32_t)v0 + 1);
+      for (uint64_t k = first_out; k < d0; ++k) {
+        const uint64_t r = T->outoff[v0] + (k - first_out);
+        if (r < T->begin || r >= T->end) continue;
+        const uint32_t v1 = n0[k];
+        const uint32_t *n1 = g->col + g->rowoff[v1];
+        const uint64_t d1 = g->rowoff[v1 + 1] - g->rowoff[v1];
+        const uint64_t mn = d0 < d1 ? d0 : d1;
+        if (cap < mn + 1) {
+          cap = 2 * (mn + 1);
+          buf = realloc(buf, sizeof(uint32_t) * cap);
+        }
+        const uint64_t c = intersect(n0, d0, n1, d1, buf);
+        const uint64_t x = lower_bound_u32(buf, 0, c, v1), p = c - x;
+        const uint32_t *P = buf + x;
+        uint64_t xq = 0;
+        for (uint64_t i = 0; i < x && p; ++i) {
+          int hit = 0;
+          FOR_EACH_P_IN_NU(g, buf[i], P, p, hit, hit = 1);
+          xq += hit;
+        }
+        uint64_t sd = 0;
+        for (uint64_t j = 0; j < p; ++j) sd += g->rowoff[P[j] + 1] - g->rowoff[P[j]];
+        deg += d0 + d1;
+        pdeg += sd;
+        rows_w += (p + xq) * ((p + 63) / 64);
+        ++roots;
+      }
+    }
+  }
+  free(buf);
+  pthread_mutex_lock(&T->mu);
+  T->deg += deg;
+  T->pdeg += pdeg;
+  T->rows_w += rows_w;
+  T->roots += roots;
+  pthread_mutex_unlock(&T->mu);
+  return NULL;
+}
+
+int orc_build_terms(const orc_graph *g, uint64_t root_begin, uint64_t root_end, int threads, uint64_t out[4]) {
+  terms_t T;
+  memset(&T, 0, sizeof(T));
+  T.g = g;
+  T.begin = root_begin;
+  T.end = root_end;
+  uint64_t *oo = malloc(sizeof(uint64_t) * (g->n + 1));
+  if (!oo) return ENOMEM;
+  oo[0] = 0;
+  for (uint64_t v = 0; v < g->n; ++v) {
+    const uint64_t d = g->rowoff[v + 1] - g->rowoff[v];
+    oo[v + 1] = oo[v] + (d - lower_bound_u32(g->col + g->rowoff[v], 0, d, (uint32_t)v + 1));
+  }
+  T.outoff = oo;
+  atomic_init(&T.next_v, 0);
+  pthread_mutex_init(&T.mu, NULL);
+  if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  pthread_t *th = malloc(sizeof(pthread_t) * (size_t)threads);
+  for (int i = 0; i < threads; ++i) pthread_create(&th[i], NULL, terms_worker, &T);
+  for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+  free(th);
+  free(oo);
+  pthread_mutex_destroy(&T.mu);
+  out[0] = T.roots;
+  out[1] = T.deg;
+  out[2] = T.pdeg;
+  out[3] = T.rows_w;
+  return 0;
+}
+
 static void *worker(void *arg) {
   thr_t *t = arg;
   shared_t *sh = t->sh;

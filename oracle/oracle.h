@@ -91,6 +91,10 @@ void orc_result_free(orc_result *r);
 
 uint64_t orc_splitmix64(uint64_t x);
 uint64_t orc_num_roots(const orc_graph *g, int root_mode);
+/* Algorithmic build terms of the edge roots [root_begin, root_end): out =
+ * {roots, sum d(v0) + d(v1), sum over P0 members of their degree, sum of
+ * rows * ceil(p / 64)} (B_build = 4 out[1] + 4 out[2] + 8 out[3]). */
+int orc_build_terms(const orc_graph *g, uint64_t root_begin, uint64_t root_end, int threads, uint64_t out[4]);
 
 #ifdef __cplusplus
 }

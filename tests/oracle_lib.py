@@ -49,6 +49,7 @@ def lib():
         L.orc_mce_tomita.argtypes = [POINTER(OrcGraph), PR]
         L.orc_mce_roots.argtypes = [POINTER(OrcGraph), c_int, c_int, POINTER(c_uint64), c_uint64, PR]
         L.orc_result_free.argtypes = [POINTER(OrcResult)]
+        L.orc_build_terms.argtypes = [POINTER(OrcGraph), c_uint64, c_uint64, c_int, POINTER(c_uint64)]
         L.orc_splitmix64.argtypes = [c_uint64]
         L.orc_splitmix64.restype = c_uint64
         L.orc_num_roots.argtypes = [POINTER(OrcGraph), c_int]
@@ -151,6 +152,19 @@ class OracleGraph:
             return _convert(r.contents, root_nodes, list_cap)
         finally:
             lib().orc_result_free(r)
+
+    def build_terms(self, root_begin=0, root_end=None, threads=0):
+        """{roots, deg, pdeg, rows_w, build_bytes} of the edge roots
+        [root_begin, root_end): the algorithmic build bytes 4 deg + 4 pdeg +
+        8 rows_w (d(v0) + d(v1), the degrees of P0's members, rows x words of
+        each root's bitmap)."""
+        end = self.p.contents.m if root_end is None else root_end
+        out = (c_uint64 * 4)()
+        if lib().orc_build_terms(self.p, root_begin, end, threads, out) != 0:
+            raise MemoryError("orc_build_terms")
+        roots, deg, pdeg, rows_w = (int(v) for v in out)
+        return {"roots": roots, "deg": deg, "pdeg": pdeg, "rows_w": rows_w,
+                "build_bytes": 4 * deg + 4 * pdeg + 8 * rows_w}
 
     def mce_roots(self, ids, root_mode=2, threads=0):
         ids = np.ascontiguousarray(ids, dtype=np.uint64)

@@ -234,3 +234,32 @@ def test_planted_mistakes_are_caught():
     for mistake in ["drop_x", "no_aggressive", "no_leaf_check", "pivot_max"]:
         h2, hs2, nodes2 = _py_canonical(n, edges, mistake, labels=lb)
         assert (h2, hs2, nodes2) != (h, hs, nodes), mistake
+
+
+@pytest.mark.parametrize("n,p,seed", [(60, 0.3, 1), (90, 0.5, 2), (200, 0.05, 3)])
+def test_build_terms_brute_force(n, p, seed):
+    """The algorithmic build terms (d(v0) + d(v1), degrees of P0's members,
+    rows x words) against a set-based recount of every edge root, and the
+    root ranges add up."""
+    from rdmce_amd.graph import Graph, OrderType
+    g = Graph.from_edges(random_graph(n, p, seed))
+    g.SortByOrder(OrderType.DEG)
+    og = OracleGraph.from_csr(g.GetRowOffset(), g.GetColIdx(), g.GetLabels(), g.GetDegeneracy())
+    ro, col, nv = g.GetRowOffset(), g.GetColIdx(), g.GetNumVertices()
+    N = [set(col[ro[v]:ro[v + 1]].tolist()) for v in range(nv)]
+    deg = pdeg = rw = roots = 0
+    for v0 in range(nv):
+        for v1 in sorted(x for x in N[v0] if x > v0):
+            c = N[v0] & N[v1]
+            P = {x for x in c if x > v1}
+            xq = sum(1 for x in c if x < v1 and N[x] & P) if P else 0
+            deg += len(N[v0]) + len(N[v1])
+            pdeg += sum(len(N[u]) for u in P)
+            rw += (len(P) + xq) * ((len(P) + 63) // 64)
+            roots += 1
+    t = og.build_terms(threads=3)
+    assert (t["roots"], t["deg"], t["pdeg"], t["rows_w"]) == (roots, deg, pdeg, rw)
+    assert t["build_bytes"] == 4 * deg + 4 * pdeg + 8 * rw
+    m = roots
+    a, b = og.build_terms(0, m // 3), og.build_terms(m // 3, m)
+    assert all(a[k] + b[k] == t[k] for k in ("roots", "deg", "pdeg", "rows_w"))
