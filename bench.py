@@ -125,12 +125,17 @@ def same(a, b) -> bool:
     return (a.count, a.hash, a.total_nodes) == (b.count, b.hash, b.total_nodes) and np.array_equal(a.hist, b.hist)
 
 
-def roofline(out, kernel_ms: float, workload: str):
+def roofline(out, kernel_ms: float, workload: str, plain=None):
     """Three ceilings of this integer set work (no MFMA): the subproblem
     builds' bytes against HBM, the search's bitmap words against LDS, and
     the search's word operations against VALU.  T_lb is the largest of the
     three times; frac = T_lb / measured kernel time, and `bound` names the
-    binding ceiling, whose achieved rate and peak are reported."""
+    binding ceiling, whose achieved rate and peak are reported.  The build
+    bytes are the ones the measured kernels count for their own builds (the
+    packed search builds one bitmap per first vertex, so they fall as builds
+    are shared).  Beside them: the per-root builds of the one-root-per-wave
+    kernel on the same roots (`plain`), counted the same way, and the per-root
+    formula with whole adjacency lists, summed by the oracle."""
     t = kernel_ms / 1e3
     b_build, b_search = out.build_bytes, out.search_bytes
     ops = OPS_PER_WORD * (b_search // 8)
@@ -144,6 +149,9 @@ def roofline(out, kernel_ms: float, workload: str):
             "t_lb_ms": t_lb * 1e3, "build_bytes": b_build, "search_bytes": b_search, "search_ops": ops,
             "ceilings_ms": {k: v[0] * 1e3 for k, v in ceil.items()},
             "frac_of_peak": {k: v[1] / v[2] for k, v in ceil.items()},
+            "per_root_build_counted": None if plain is None else {
+                "build_bytes": plain.build_bytes, "hbm_ms": plain.build_bytes / HBM_PEAK_GBS / 1e6,
+                "frac": max(plain.build_bytes / HBM_PEAK_GBS / 1e9, ceil["lds"][0], ceil["valu"][0]) / t},
             "per_root_build_formula": per_root_build(workload, t)}
 
 
@@ -407,7 +415,7 @@ def main():
             "value": abl_outs[0].count * a_steps / abl_wall, "ms_per_step": abl_wall * 1e3 / a_steps,
             "kernel_ms": sum(o.kernel_ms for o in abl_outs) / a_steps, "steps": a_steps},
             "learned_gain": (abl_wall / a_steps) / (wall / args.steps)},
-        "roofline": roofline(first, kernel_ms, args.workload),
+        "roofline": roofline(first, kernel_ms, args.workload, plain),
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(g, w.n_roots, args.cpu_seconds, prep_s)
