@@ -601,7 +601,7 @@ mce_status plan_create_impl(const PlanSpec &spec, const mce_options &opt, mce_pl
     // (the packed search's pool bound)
     for (uint64_t v = 0; v < g.n; ++v) {
       const uint64_t d = g.rowoff[v + 1] - outoff[v], x = outoff[v] - g.rowoff[v];
-      if (d >= 2 && d <= 1024) p->pool_bound += 8 + (d + 63) / 64 * (align_up(d + x, 64) + 1);
+      if (d >= 2 && d <= 1024) p->pool_bound += 8 + (d + 63) / 64 * (align_up(d + x, 64) + 1) + d;
     }
     PTRY(dalloc(&p->d_outoff, g.n + 1));
     PTRY(hipMemcpy(p->d_outoff, outoff.data(), (g.n + 1) * 8, hipMemcpyHostToDevice));

@@ -289,7 +289,7 @@ __device__ __forceinline__ uint32_t pk_build(const PackWave<NW, NX> &w, uint8_t 
 // exactly the one pk_build makes, so the search tree is unchanged.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kGroupMaxW = 16;  // |U| <= 1024
-constexpr uint32_t kPoolHdr = 8;     // kept vertex bitmap: v0 d0 W0 xq0 rp o0 rfirst, then T and cm
+constexpr uint32_t kPoolHdr = 8;     // kept vertex bitmap: v0 d0 W0 xq0 rp o0 rfirst lh0, then T, cm and lh
 
 struct PkGroup {
   uint32_t v0, d0, W0, xq0, rp;
@@ -298,6 +298,8 @@ struct PkGroup {
   const uint64_t *T;    // W0 x rp words: rows 0..d0-1 (U), d0..d0+xq0-1 (X0v')
   const uint64_t *cm;   // W0 words: OR of the X0v' rows (U vertices with a neighbour below v0)
   uint64_t *wide;       // the run's roots too wide for this variant, kept until the run ends
+  const uint64_t *lh;   // d0 words: splitmix64(label) of each member of U
+  uint64_t lh0;         // splitmix64(label(v0))
 };
 
 // Builds the bitmap of vertex root v0 into `scratch` (the whole wave), with
@@ -315,7 +317,8 @@ __device__ __forceinline__ bool pk_group_build(const KArgs &a, uint8_t *scratch,
   const uint64_t rp_max = align_up(d0 + x0, 64);
   const uint64_t cm_off = t_off + align_up(8 * W0 * rp_max, 256);
   const uint64_t wide_off = cm_off + align_up(8 * W0, 256);
-  if (wide_off + 8 * run > a.slot_bytes) return false;
+  const uint64_t lh_off = wide_off + align_up(8 * run, 256);
+  if (lh_off + 8 * d0 > a.slot_bytes) return false;
   uint32_t *Xs = (uint32_t *)scratch;
   for (uint64_t i = l; i < x0; i += 64) Xs[i] = a.col[b0 + i];
   wave_fence();
@@ -352,6 +355,11 @@ __device__ __forceinline__ bool pk_group_build(const KArgs &a, uint8_t *scratch,
   G.T = T;
   G.cm = cm;
   G.wide = (uint64_t *)(scratch + wide_off);
+  uint64_t *lh = (uint64_t *)(scratch + lh_off);
+  for (uint64_t j = l; j < d0; j += 64) lh[j] = a.label_hash[a.col[o0 + j]];
+  wave_fence();
+  G.lh = lh;
+  G.lh0 = a.label_hash[v0];
   return true;
 }
 
@@ -389,9 +397,9 @@ __device__ __forceinline__ uint32_t pk_derive(const PackWave<NW, NX> &w, const P
   const KArgs &a = w.a;
   const uint32_t l = lane();
   const uint32_t k = (uint32_t)(canon - G.rfirst);
-  const uint32_t uk = a.col[G.o0 + k];
+  const uint32_t uk = a.list ? a.col[G.o0 + k] : 0u;  // the root's v1 (clique lists only)
   const uint32_t graph = a.root_graph ? a.root_graph[canon] : 0;
-  const uint64_t base_key = a.label_hash[G.v0] + a.label_hash[uk];
+  const uint64_t base_key = G.lh0 + G.lh[k];
   // Row k of U, split at k: P0 above it, the U part of X0 below it (one
   // word per lane).
   const uint32_t q = l;
@@ -519,9 +527,8 @@ __device__ __forceinline__ uint32_t pk_derive(const PackWave<NW, NX> &w, const P
     if (t < p) {
 #pragma unroll
       for (int z = 0; z < NW; ++z) sl[C::sProw + t * NW + z] = o[z];
-      const uint32_t u = a.col[G.o0 + prow_id[r]];
-      sl[C::sG + t] = a.label_hash[u];
-      if (a.list) a.pk_plist[((uint64_t)w.me * w.S + s) * kPackMaxP + t] = u;
+      sl[C::sG + t] = G.lh[prow_id[r]];
+      if (a.list) a.pk_plist[((uint64_t)w.me * w.S + s) * kPackMaxP + t] = a.col[G.o0 + prow_id[r]];
     }
   }
 #pragma unroll
@@ -609,7 +616,7 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
       if (gcur == gend) {
         if (nwide) {  // the finished run's wide roots go out with a copy of its vertex bitmap
           wave_fence();
-          const uint64_t tw = (uint64_t)G.W0 * G.rp, need = kPoolHdr + tw + G.W0;
+          const uint64_t tw = (uint64_t)G.W0 * G.rp, need = kPoolHdr + tw + G.W0 + G.d0;
           unsigned long long at = 0, hp = 0;
           if (l == 0) {
             at = atomicAdd(a.n_wide, (unsigned long long)nwide);
@@ -621,7 +628,7 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
           if (kept) {
             uint64_t *dst = a.pk_pool + hp;
             if (l < kPoolHdr) {
-              const uint64_t hv[kPoolHdr] = {G.v0, G.d0, G.W0, G.xq0, G.rp, G.o0, G.rfirst, 0};
+              const uint64_t hv[kPoolHdr] = {G.v0, G.d0, G.W0, G.xq0, G.rp, G.o0, G.rfirst, G.lh0};
               uint64_t v = 0;
 #pragma unroll
               for (uint32_t f = 0; f < kPoolHdr; ++f) v = f == l ? hv[f] : v;
@@ -629,6 +636,7 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
             }
             for (uint64_t i = l; i < tw; i += 64) dst[kPoolHdr + i] = G.T[i];
             for (uint64_t i = l; i < G.W0; i += 64) dst[kPoolHdr + tw + i] = G.cm[i];
+            for (uint64_t i = l; i < G.d0; i += 64) dst[kPoolHdr + tw + G.W0 + i] = G.lh[i];
             acc.build += 8 * need;
           }
           for (uint32_t i = l; i < nwide; i += 64) {
@@ -671,8 +679,10 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
         G.rp = (uint32_t)hdr[4];
         G.o0 = hdr[5];
         G.rfirst = hdr[6];
+        G.lh0 = hdr[7];
         G.T = hdr + kPoolHdr;
         G.cm = G.T + (uint64_t)G.W0 * G.rp;
+        G.lh = G.cm + G.W0;
         derive = true;
       }
       uint32_t pruned = 0;
