@@ -107,6 +107,30 @@ def test_choose_root_mode(pack):
     assert ch["sample_cliques"] == ref.count
 
 
+EDGE_CASES = {"one edge": [(0, 1)], "matching": [(2 * i, 2 * i + 1) for i in range(40)],
+              "star": [(0, i) for i in range(1, 70)], "path": [(i, i + 1) for i in range(100)],
+              "two triangles on an edge": [(0, 1), (1, 2), (0, 2), (1, 3), (0, 3)]}
+
+
+@pytest.mark.parametrize("name", sorted(EDGE_CASES))
+def test_edge_case_graphs_packed(name):
+    """Graphs whose roots are all 2-cliques, single triangles or empty P:
+    the packed search settles them during the build, as the oracle does."""
+    banner(f"pack edge case {name}")
+    g = R.Graph.from_edges(EDGE_CASES[name])
+    g.SortByOrder(R.OrderType.DEG)
+    out = R.solve(g, pack=True, want_root_stats=True, want_list=True)
+    check_same(out, oracle_for(g, list_cap=1 << 12), lists=True)
+
+
+def test_empty_graph_packed():
+    banner("pack empty graph")
+    g = R.Graph.from_edges([])
+    assert g.GetNumVertices() == 0
+    out = R.solve(g, pack=True)
+    assert (out.count, out.hist_dict()) == (0, {})
+
+
 def test_er_config1_packed():
     banner("pack ER G(2048, 0.008) seed 1")
     g = R.Graph.erdos_renyi(2048, 0.008, 1)
