@@ -796,6 +796,10 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
         if (base == (uint32_t)q * 64) rw = rem[q];
       t = base + select64(rw, kk);
     }
+    // A child's P | XP lies within its parent's P | XP: when no parent of
+    // this pass reaches the last word (a root's top node may), the tasks
+    // skip that word of every row they scan.
+    const bool wlast = NW == 1 || __ballot(act && ((meta & kRecRoot) || (P[NW - 1] | XP[NW - 1]) != 0)) != 0;
     wave_fence();  // every parent is read before records are rewritten below
     if (lpart) {   // the partly taken record keeps its candidates from the (T - excl)-th on
       uint32_t kk = T - excl, base = 0;
@@ -886,23 +890,27 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
               uint32_t c0 = 0, c1 = 0;
 #pragma unroll
               for (int z = 0; z < NW; ++z) {
-                r0[z] = sl[C::sProw + i0 * NW + z];
-                r1[z] = sl[C::sProw + i1 * NW + z];
+                r0[z] = z + 1 < NW || wlast ? (uint64_t)sl[C::sProw + i0 * NW + z] : 0ull;
+                r1[z] = z + 1 < NW || wlast ? (uint64_t)sl[C::sProw + i1 * NW + z] : 0ull;
               }
 #pragma unroll
               for (int z = 0; z < NW; ++z) {
-                c0 += (uint32_t)__popcll(P[z] & ~r0[z]);
-                c1 += (uint32_t)__popcll(P[z] & ~r1[z]);
+                if (z + 1 < NW || wlast) {
+                  c0 += (uint32_t)__popcll(P[z] & ~r0[z]);
+                  c1 += (uint32_t)__popcll(P[z] & ~r1[z]);
+                }
               }
               if (c0 == 1) {
                 mk[q] |= b0;
 #pragma unroll
-                for (int z = 0; z < NW; ++z) xacc[z] &= r0[z];
+                for (int z = 0; z < NW; ++z)
+                  if (z + 1 < NW || wlast) xacc[z] &= r0[z];
               }
               if (b1 && c1 == 1) {
                 mk[q] |= b1;
 #pragma unroll
-                for (int z = 0; z < NW; ++z) xacc[z] &= r1[z];
+                for (int z = 0; z < NW; ++z)
+                  if (z + 1 < NW || wlast) xacc[z] &= r1[z];
               }
             }
           }
@@ -965,14 +973,16 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
             uint64_t r0[NW], r1[NW];
 #pragma unroll
             for (int z = 0; z < NW; ++z) {
-              r0[z] = sl[rbase + u0 * NW + z];
-              r1[z] = sl[rbase + u1 * NW + z];
+              r0[z] = z + 1 < NW || wlast ? (uint64_t)sl[rbase + u0 * NW + z] : 0ull;
+              r1[z] = z + 1 < NW || wlast ? (uint64_t)sl[rbase + u1 * NW + z] : 0ull;
             }
             uint32_t c0 = 0, c1 = 0;
 #pragma unroll
             for (int z = 0; z < NW; ++z) {
-              c0 += (uint32_t)__popcll(P[z] & ~r0[z]);
-              c1 += (uint32_t)__popcll(P[z] & ~r1[z]);
+              if (z + 1 < NW || wlast) {
+                c0 += (uint32_t)__popcll(P[z] & ~r0[z]);
+                c1 += (uint32_t)__popcll(P[z] & ~r1[z]);
+              }
             }
             const uint32_t k0 = (c0 << 16) | (kbase + u0), k1 = (c1 << 16) | (kbase + u1);
             best = k0 < best ? k0 : best;
