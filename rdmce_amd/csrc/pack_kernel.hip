@@ -36,7 +36,9 @@
 // appended to a.wide for the next launch of the run.  When a.wide_h is set,
 // the vertex bitmap of a run that met such roots is copied to a.pk_pool
 // (while it has room) and the next launch derives each of them from that
-// copy (a.pk_whandle) instead of building it again.
+// copy (a.pk_whandle) instead of building it again.  A derived root with
+// too many X0' rows whose top node one of them prunes (it covers P0) is
+// settled there, as one tree node, and goes nowhere.
 #include "mce_common.hpp"
 
 extern __shared__ uint64_t lds_dyn[];
