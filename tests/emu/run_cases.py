@@ -185,6 +185,19 @@ def cases():
         return same(out, ref)
     yield "pack routed runs", pack_routed_runs
 
+    def choose_root_mode():
+        # vertex roots of every 3rd vertex against those vertices' edge
+        # roots: the chooser checks they agree; the oracle checks the count
+        rg = graph(random_graph(120, 0.1, 21))
+        ref = OracleGraph.from_csr(rg.GetRowOffset(), rg.GetColIdx(), rg.GetLabels(), rg.GetDegeneracy()).mce_roots(
+            np.arange(0, rg.GetNumVertices(), 3, dtype=np.uint64), root_mode=1)
+        for pack in (False, True):
+            ch = R.choose_root_mode(rg, stride=3, pack=pack, waves_per_cu=wpc, pack_waves_per_cu=2)
+            if ch["sample_cliques"] != ref.count or ch["root_mode"] not in (R.ROOT_VERTEX, R.ROOT_EDGE):
+                return f"chooser pack={pack}: {ch} against {ref.count} cliques"
+        return None
+    yield "choose root mode", choose_root_mode
+
     def pack_batch():
         b = R.Batch(bgs, waves_per_cu=wpc, pack=True, pack_waves_per_cu=2)
         out = b.run()

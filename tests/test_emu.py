@@ -17,7 +17,8 @@ QUICK = ["zachary", "closed K5", "closed moon_moser4", "closed cocktail8", "clos
          "G(60,0.5) tau=1 lds=1", "G(60,0.5) tau=-1 lds=0", "G(120,0.3) tau=4 lds=0", "G(200,0.08) tau=24 lds=1",
          "vertex roots", "two-word subset", "deferred pass", "batch", "plan graph replay",
          "pack zachary list", "pack G(60,0.5) slots=1", "pack G(45,0.8) slots=4", "pack wide runs K90",
-         "pack wide runs K150", "pack routed runs", "pack batch"]
+         "pack wide runs K150", "pack routed runs", "pack batch",
+         "choose root mode"]
 
 
 @pytest.fixture(scope="module")
