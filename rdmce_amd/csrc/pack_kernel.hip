@@ -106,7 +106,7 @@ struct PackWave {
 
 // Counts one maximal clique of slot s (size, key = sum of label hashes),
 // with R's members given by rb (bits over P0).  Per lane.
-template <int NW, int NX>
+template <int NW, int NX, bool LISTS>
 __device__ __forceinline__ void pk_emit(const PackWave<NW, NX> &w, uint32_t s, uint32_t size, uint64_t key,
                                         const uint64_t (&rb)[NW], uint64_t &cnt, uint64_t &hsh) {
   using C = PackCfg<NW, NX>;
@@ -128,7 +128,7 @@ __device__ __forceinline__ void pk_emit(const PackWave<NW, NX> &w, uint32_t s, u
       atomicOr(a.err, kErrHistOverflow);
     }
   }
-  if (a.list) {
+  if (LISTS && a.list) {
     // entry: size, the root's vertices, then R's members in P0 order
     const unsigned long long pos = atomicAdd(a.list_cursor, (unsigned long long)(size + 1));
     if (pos + size + 1 > a.list_cap) {
@@ -573,7 +573,7 @@ __device__ __forceinline__ void pk_finish(const PackWave<NW, NX> &w, uint32_t s)
   }
 }
 
-template <int NW, int NX>
+template <int NW, int NX, bool LISTS>
 __device__ __forceinline__ void pack_main(const KArgs &a) {
   using C = PackCfg<NW, NX>;
   const uint32_t l = lane();
@@ -595,7 +595,7 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
   // neighbouring words.
   uint64_t *stk = a.pk_stack + (uint64_t)w.me * a.pk_cap * C::RS;
   const uint64_t cap = a.pk_cap;
-  const bool lists = a.list != nullptr;
+  constexpr bool lists = LISTS;  // clique lists: R is kept in the records
   auto fld = [stk, cap](uint32_t f, uint32_t r) -> uint64_t & { return stk[(uint64_t)f * cap + r]; };
   ldsu32 *map32 = (ldsu32 *)w.L.map;
   Acc acc{0, 0, 0, 0, 0};
@@ -956,7 +956,7 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
       ++nodes;
       if (a.root_nodes) __hip_atomic_fetch_add(w.m(mNodes, s), (uint64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (psz == 0) {
-        if (xany == 0) pk_emit<NW, NX>(w, s, (uint32_t)*w.m(mBase, s) + rsize, key, R, cnt, hsh);
+        if (xany == 0) pk_emit<NW, NX, LISTS>(w, s, (uint32_t)*w.m(mBase, s) + rsize, key, R, cnt, hsh);
       } else {
         words += (uint64_t)(cw + xw) * Wr;
         // pivot: (|P' \ N(u)|, row) argmin; rows of P0 first, then X0'
@@ -1062,21 +1062,25 @@ __device__ __forceinline__ void pack_main(const KArgs &a) {
 
 // One kernel per variant and occupancy target (WPE waves per SIMD: the
 // register budget 512 / WPE VGPRs).
-template <int NW, int NX, int WPE>
+template <int NW, int NX, int WPE, bool LISTS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void pack_kernel(KArgs a) {
-  pack_main<NW, NX>(a);
+  pack_main<NW, NX, LISTS>(a);
 }
 
 template <int NW, int NX>
 hipError_t launch_variant(const KArgs &a, uint32_t waves, uint32_t wpc, size_t lds, hipStream_t stream) {
-  if (wpc <= 16)
-    hipLaunchKernelGGL((pack_kernel<NW, NX, 4>), dim3(waves), dim3(64), lds, stream, a);
+  // Runs that write clique lists carry R in every record: one instantiation
+  // (4 waves per SIMD) for them, so the others hold no R at all.
+  if (a.list)
+    hipLaunchKernelGGL((pack_kernel<NW, NX, 4, true>), dim3(waves), dim3(64), lds, stream, a);
+  else if (wpc <= 16)
+    hipLaunchKernelGGL((pack_kernel<NW, NX, 4, false>), dim3(waves), dim3(64), lds, stream, a);
   else if (wpc <= 20)
-    hipLaunchKernelGGL((pack_kernel<NW, NX, 5>), dim3(waves), dim3(64), lds, stream, a);
+    hipLaunchKernelGGL((pack_kernel<NW, NX, 5, false>), dim3(waves), dim3(64), lds, stream, a);
   else if (wpc <= 24)
-    hipLaunchKernelGGL((pack_kernel<NW, NX, 6>), dim3(waves), dim3(64), lds, stream, a);
+    hipLaunchKernelGGL((pack_kernel<NW, NX, 6, false>), dim3(waves), dim3(64), lds, stream, a);
   else
-    hipLaunchKernelGGL((pack_kernel<NW, NX, 8>), dim3(waves), dim3(64), lds, stream, a);
+    hipLaunchKernelGGL((pack_kernel<NW, NX, 8, false>), dim3(waves), dim3(64), lds, stream, a);
   return hipGetLastError();
 }
 
